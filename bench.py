@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark: AMICA EM iterations per second on one MI355X.
+
+Workload (default): 128 channels x 2,000,000 samples, M=4 models, m=3
+mixture components, non-stationary synthetic data (4 mixing matrices over
+50k-sample blocks), BF16 GEMM operands with FP32 density math and FP64
+statistics / M-step.  A step is one full EM iteration (unmixing GEMM, fused
+E-step, statistic GEMM, reductions, M-step with Newton/natural gradient and
+reparameterisation) over all samples, which stay resident in HBM.
+
+Prints one JSON line.  Fields beyond the driver's contract:
+  roofline      HBM roofline of the fused E-step, from HIP events recorded
+                around it on the engine stream during the timed steps
+  cpu_baseline  the FP64 CPU oracle on a bounded sample of the workload
+  ll_rel_err_vs_fp64  |LL_gpu - LL_fp64| / |LL_fp64| over the first
+                iterations on that sample (same data, same initial state)
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "EM iters/s @128ch×2M samples,M=4; % HBM roofline; rel. ΔLL vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def estep_bytes(n, T, M, precision):
+    """Algorithmic HBM bytes of one E-step launch: M n T (b read + g written)."""
+    g_bytes = 2 if precision == 16 else 4
+    return M * n * T * (4 + g_bytes)
+
+
+def cpu_baseline(w, x, st0, precision, iters=2, target_s=15.0):
+    """The FP64 oracle on the first T_s samples (T_s sized for ~target_s of
+    CPU work), and the GPU engine on the same samples for the LL check."""
+    import cudamica_amd as ca
+    from oracle import pyoracle as po
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    n, M, m = w.n, w.M, w.m
+    # calibrate on a tiny slice, then size the sample
+    Tc = min(w.T, 20000)
+    xw, S, mean, ldetS = po.preprocess(x[:Tc], w.do_sphere)
+    cfg = po.default_config(n, Tc, M, m, maxiter=1, newt_start_iter=w.newt_start_iter)
+    t0 = time.perf_counter()
+    po.run(cfg, xw, ldetS, st0, nthreads=cores)
+    per_sample = (time.perf_counter() - t0) / Tc
+    Ts = int(min(w.T, max(Tc, target_s / iters / max(per_sample, 1e-12))))
+    Ts -= Ts % 4
+    log(f"cpu baseline: oracle FP64, {cores} threads, {iters} iterations on {Ts} samples")
+    xs = x[:Ts]
+    xw, S, mean, ldetS = po.preprocess(xs, w.do_sphere)
+    cfg = po.default_config(n, Ts, M, m, maxiter=iters, newt_start_iter=w.newt_start_iter)
+    t0 = time.perf_counter()
+    fin, logs = po.run(cfg, xw, ldetS, st0, nthreads=cores)
+    dt = time.perf_counter() - t0
+    its_full = 1.0 / ((dt / iters) * (w.T / Ts))
+    with ca.Engine(w.options(T=Ts, precision=precision, max_iter=iters)) as eng:
+        eng.load_raw(xs)
+        eng.preprocess(w.do_sphere)
+        eng.set_state(st0)
+        eng.iterate(iters)
+        eng.sync()
+        ll_gpu = eng.log(iters)["ll"]
+    rel = float(np.max(np.abs(ll_gpu - logs["ll"]) / np.abs(logs["ll"])))
+    return {"value": its_full, "unit": "EM iters/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, {iters} EM iterations on the first {Ts} of {w.T} samples, "
+                      f"{dt:.1f} s, scaled by T/{Ts}"}, rel, Ts
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="n128_T2M_M4")
+    ap.add_argument("--precision", choices=["fp32", "bf16"], default=None)
+    ap.add_argument("--T", type=int, default=None, help="override the sample count")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this engine does not split samples across GPUs "
+              "(one GPU owns all samples); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import cudamica_amd as ca
+    w = ca.WORKLOADS[args.workload]
+    if args.T:
+        w.T = args.T
+    prec = w.precision if args.precision is None else (16 if args.precision == "bf16" else 32)
+    n, T, M, m = w.n, w.T, w.M, w.m
+
+    t0 = time.perf_counter()
+    x = ca.synth_data(w.spec())
+    st0 = w.init_state()
+    log(f"data {T}x{n} generated in {time.perf_counter() - t0:.1f} s")
+
+    eng = ca.Engine(w.options(precision=prec, max_iter=args.warmup + args.steps + 1))
+    eng.load_raw(x)
+    t0 = time.perf_counter()
+    eng.preprocess(w.do_sphere)
+    log(f"preprocessing on the GPU: {time.perf_counter() - t0:.2f} s")
+    eng.set_state(st0)
+    eng.iterate(args.warmup)
+    eng.sync()
+    log(f"warmup {args.warmup} iterations done")
+
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    eng.iterate(args.steps)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    kms, cnt = eng.timing()
+    eng.set_timing(False)
+    lg = eng.log(args.warmup + args.steps)
+    eng.close()
+    its = args.steps / dt
+    log(f"timed {args.steps} iterations: {dt * 1e3 / args.steps:.3f} ms/iter; kernels (ms) "
+        f"prepare+unmix {kms[0]:.3f} estep {kms[1]:.3f} gx {kms[2]:.3f} reduce+mstep {kms[3]:.3f}; "
+        f"LL {lg['ll'][-1]:.10f}")
+
+    nbytes = estep_bytes(n, T, M, prec)
+    achieved = nbytes / (kms[1] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": its,
+        "unit": "EM iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16+f32" if prec == 16 else "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "n": n, "T": T, "M": M, "m": m,
+                   "precision": "bf16" if prec == 16 else "fp32", "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "estep_kernel", "bytes_per_launch": nbytes,
+                     "avg_ms": kms[1], "launches": cnt},
+        "kernel_ms": {"prepare_unmix": kms[0], "estep": kms[1], "stat_gemm": kms[2],
+                      "reduce_mstep": kms[3]},
+        "ll_last": float(lg["ll"][-1]),
+    }
+    if not args.no_cpu_baseline:
+        cb, rel, Ts = cpu_baseline(w, x, st0, prec)
+        out["cpu_baseline"] = cb
+        out["ll_rel_err_vs_fp64"] = rel
+        out["ll_check_samples"] = Ts
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

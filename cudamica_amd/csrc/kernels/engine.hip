@@ -1,0 +1,651 @@
+/*
+ * engine.hip -- host side of the MI355X AMICA engine (C ABI in
+ * include/amica_engine.h).
+ *
+ * One engine owns one recording on one GPU.  Device memory (sizes for
+ * n channels padded to NP = 64*ceil(n/64), ldb = M*NP):
+ *   x        float32 [T][NP]   whitened data (+ bf16 copy in BF16 mode)
+ *   b        float32 [T][ldb]  unmixed data of all models (b = W x - W c)
+ *   g        fp32/bf16 [T][ldb] score g = sum_j sqrt(beta) u f
+ *   part     FP64 [nblocks][pstride] E-step partial statistics
+ *   gxpart   FP64 [nchunks][ldb][NP] partials of G = sum_t g x^T
+ *   params   FP64 A, c, alpha, mu, beta, rho, gm, W, ldet (+ GEMM copy of W)
+ * One EM iteration is eight launches on one stream, with every decision of
+ * the reference's host loop (LL decrease, Newton test, lrate schedule) made
+ * on the device, so an iteration can be replayed as one hipGraph.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/amica_engine.h"
+#include "common.h"
+#include "engine_internal.h"
+
+using namespace amica;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                  \
+    do {                                                                              \
+        hipError_t _e = (call);                                                       \
+        if (_e != hipSuccess)                                                         \
+            return fail(AMICA_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define LCHK(call)                                                                    \
+    do {                                                                              \
+        if ((call) != 0) {                                                            \
+            hipError_t _e = hipGetLastError();                                        \
+            return fail(AMICA_E_DEVICE, std::string("launch failed: ") + #call + ": " + \
+                                            hipGetErrorString(_e));                   \
+        }                                                                             \
+    } while (0)
+
+long canonical_stride(int n, int m) { return 1 + 8L * n * m + 2L * n + (long)n * n; }
+
+}  // namespace
+
+struct amica_engine {
+    amica_options o;
+    int NP = 0, ldb = 0, CW = 0;
+    int bf16 = 0;
+    int cus = 0;
+    hipStream_t st = nullptr;
+    /* data */
+    float* xraw = nullptr;
+    float* xw = nullptr;
+    uint16_t* xbf = nullptr;
+    float* b = nullptr;
+    void* g = nullptr;
+    /* partials */
+    int nblocks = 0;
+    long chunk = 0;
+    size_t pstride = 0;
+    double* part = nullptr;
+    int nchunks = 0;
+    double* gxpart = nullptr;
+    double* raw = nullptr;
+    double* gxtot = nullptr;
+    /* parameters and work */
+    double *A = nullptr, *c = nullptr, *alpha = nullptr, *mu = nullptr, *beta = nullptr,
+           *rho = nullptr, *gm = nullptr, *W = nullptr, *ldet = nullptr, *lconst = nullptr;
+    void* Wg = nullptr;
+    float* wc = nullptr;
+    float* cst = nullptr;
+    double* stats = nullptr;
+    long stride = 0;
+    double *sigma2 = nullptr, *cnew = nullptr, *phis = nullptr, *kappa = nullptr,
+           *lambda = nullptr, *Bm = nullptr, *work = nullptr, *lu = nullptr;
+    int *bflag = nullptr, *newton = nullptr;
+    DevState* ds = nullptr;
+    double *log_ll = nullptr, *log_lrate = nullptr, *log_vsum = nullptr;
+    int *log_bflag = nullptr, *log_newton = nullptr, *log_lldec = nullptr;
+    float* ltall = nullptr;
+    int ltall_on = 0;
+    /* preprocessing */
+    double *S = nullptr, *mean = nullptr, *ldetS_d = nullptr;
+    double ldetS = 0.0;
+    int have_data = 0, have_mean = 0, have_white = 0, have_state = 0;
+    /* graph */
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    /* timing: one set of 5 events per timed iteration, read back lazily */
+    int timing = 0;
+    std::vector<std::array<hipEvent_t, 5>> evs;
+    size_t ev_used = 0;
+    hipEvent_t* ev = nullptr;
+
+    MstepArgs margs() const {
+        MstepArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.n = o.n; a.NP = NP; a.M = o.M; a.m = o.m; a.ldb = ldb; a.prec_bf16 = bf16;
+        a.T = o.T; a.log_cap = o.max_iter;
+        a.do_newton = o.do_newton; a.update_rho = o.update_rho;
+        a.rho_start_iter = o.rho_start_iter; a.newt_start_iter = o.newt_start_iter;
+        a.update_gm = o.update_gm; a.update_alpha = o.update_alpha; a.update_A = o.update_A;
+        a.update_mu = o.update_mu; a.update_beta = o.update_beta; a.do_reparm = o.do_reparm;
+        a.lratemax = o.lratemax; a.lnatrate = o.lnatrate; a.lratefact = o.lratefact;
+        a.rholrate = o.rholrate; a.rhomin = o.rhomin; a.rhomax = o.rhomax; a.maxdec = o.maxdec;
+        a.A = A; a.c = c; a.alpha = alpha; a.mu = mu; a.beta = beta; a.rho = rho; a.gm = gm;
+        a.W = W; a.ldet = ldet; a.lconst = lconst;
+        a.Wg = Wg; a.wc = wc; a.cst = cst;
+        a.stats = stats; a.stride = stride;
+        a.sigma2 = sigma2; a.cnew = cnew; a.phis = phis; a.kappa = kappa; a.lambda = lambda;
+        a.B = Bm; a.work = work; a.lu = lu;
+        a.bflag = bflag; a.newton = newton; a.ds = ds;
+        a.log_ll = log_ll; a.log_lrate = log_lrate; a.log_bflag = log_bflag;
+        a.log_newton = log_newton; a.log_lldec = log_lldec; a.log_vsum = log_vsum;
+        return a;
+    }
+
+    EstepArgs eargs(float* lt) const {
+        EstepArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.b = b; a.x = xw; a.g = g; a.cst = cst; a.lconst = lconst; a.part = part; a.ltall = lt;
+        a.T = o.T; a.chunk = chunk; a.pstride = pstride;
+        a.n = o.n; a.NP = NP; a.M = o.M; a.CW = CW; a.ldb = ldb; a.g_bf16 = bf16;
+        return a;
+    }
+
+    /* prepare, unmixing GEMM, E-step, statistic GEMM, reductions */
+    int enqueue_estep(bool with_timing) {
+        const MstepArgs ma = margs();
+        if (with_timing) HIPCHK(hipEventRecord(ev[0], st));
+        LCHK(launch_prepare(ma, st));
+        if (bf16)
+            LCHK(launch_unmix_bf16(xbf, (const uint16_t*)Wg, wc, b, o.T, NP, ldb, st));
+        else
+            LCHK(launch_unmix_f32(xw, (const float*)Wg, wc, b, o.T, NP, ldb, st));
+        if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
+        LCHK(launch_estep(eargs(ltall_on ? ltall : nullptr), o.m, nblocks, st));
+        if (with_timing) HIPCHK(hipEventRecord(ev[2], st));
+        if (bf16)
+            LCHK(launch_gx_bf16((const uint16_t*)g, xbf, gxpart, o.T, NP, ldb, nchunks, st));
+        else
+            LCHK(launch_gx_f32((const float*)g, xw, gxpart, o.T, NP, ldb, nchunks, st));
+        if (with_timing) HIPCHK(hipEventRecord(ev[3], st));
+        LCHK(launch_reduce(part, nblocks, pstride, gxpart, nchunks, raw, gxtot, ma, st));
+        return 0;
+    }
+
+    int enqueue_iteration(bool with_timing) {
+        int rc = enqueue_estep(with_timing);
+        if (rc) return rc;
+        LCHK(launch_mstep(margs(), st));
+        if (with_timing) HIPCHK(hipEventRecord(ev[4], st));
+        return 0;
+    }
+
+    int next_event_set() {
+        if (ev_used == evs.size()) {
+            std::array<hipEvent_t, 5> s;
+            for (auto& v : s) HIPCHK(hipEventCreate(&v));
+            evs.push_back(s);
+        }
+        ev = evs[ev_used++].data();
+        return 0;
+    }
+};
+
+extern "C" {
+
+const char* amica_last_error(void) { return g_last_error.c_str(); }
+
+void amica_options_default(amica_options* o, int n, long T, int M, int m) {
+    std::memset(o, 0, sizeof(*o));
+    o->n = n; o->T = T; o->M = M; o->m = m;
+    o->precision = AMICA_PREC_FP32;
+    o->max_iter = 200;
+    o->device = 0;
+    o->do_newton = 1; o->update_rho = 1; o->rho_start_iter = 1; o->newt_start_iter = 50;
+    o->update_gm = o->update_alpha = o->update_A = o->update_mu = o->update_beta = 1;
+    o->do_reparm = 1;
+    o->lrate0 = 0.1; o->lratemax = 1.0; o->lnatrate = 0.1; o->lratefact = 0.5;
+    o->rholrate = 0.1; o->rhomin = 1.0; o->rhomax = 2.0;
+    o->maxdec = 3;
+}
+
+int amica_engine_create(const amica_options* opts, amica_engine** out) {
+    if (!opts || !out) return fail(AMICA_E_INVALID, "null argument");
+    const amica_options& o = *opts;
+    if (o.n < 1 || o.n > 1024) return fail(AMICA_E_INVALID, "n must be in [1, 1024]");
+    if (o.T < 2) return fail(AMICA_E_INVALID, "T must be >= 2");
+    if (o.M < 1 || o.M > AMICA_MAX_M) return fail(AMICA_E_INVALID, "M must be in [1, 16]");
+    if (o.m < 1 || o.m > AMICA_MAX_MJ) return fail(AMICA_E_INVALID, "m must be in [1, 8]");
+    if (o.precision != AMICA_PREC_FP32 && o.precision != AMICA_PREC_BF16)
+        return fail(AMICA_E_INVALID, "precision must be 32 (fp32) or 16 (bf16)");
+    const int NP = ((o.n + 63) / 64) * 64;
+    const int CW = NP / 64;
+    if (o.M * CW > kMaxEstepWaves)
+        return fail(AMICA_E_INVALID, "M * ceil(n/64) must be <= 8 for the fused E-step");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (o.device < 0 || o.device >= ndev) return fail(AMICA_E_INVALID, "no such HIP device");
+    HIPCHK(hipSetDevice(o.device));
+
+    amica_engine* e = new amica_engine();
+    e->o = o;
+    if (e->o.max_iter < 1) e->o.max_iter = 1;
+    e->NP = NP;
+    e->CW = CW;
+    e->ldb = o.M * NP;
+    e->bf16 = (o.precision == AMICA_PREC_BF16);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, o.device));
+    e->cus = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+
+    const long T = o.T;
+    const int n = o.n, M = o.M, m = o.m;
+    auto alloc = [&](void** p, size_t bytes) -> int {
+        if (bytes == 0) bytes = 8;
+        hipError_t r = hipMalloc(p, bytes);
+        if (r != hipSuccess) return fail(AMICA_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        r = hipMemset(*p, 0, bytes);
+        if (r != hipSuccess) return fail(AMICA_E_DEVICE, "hipMemset failed");
+        return 0;
+    };
+#define ALLOC(ptr, bytes) do { int _r = alloc((void**)&(ptr), (bytes)); if (_r) { amica_engine_destroy(e); return _r; } } while (0)
+    ALLOC(e->xw, (size_t)T * NP * sizeof(float));
+    if (e->bf16) ALLOC(e->xbf, (size_t)T * NP * sizeof(uint16_t));
+    ALLOC(e->b, (size_t)T * e->ldb * sizeof(float));
+    ALLOC(e->g, (size_t)T * e->ldb * (e->bf16 ? sizeof(uint16_t) : sizeof(float)));
+    /* E-step grid: every CU holds 8 waves of this kernel (2 per SIMD). */
+    const int waves = M * CW;
+    const int wg_per_cu = std::max(1, 8 / waves);
+    long target = (long)e->cus * wg_per_cu;
+    long maxb = (T + 3) / 4;
+    e->nblocks = (int)std::max(1L, std::min(target, maxb));
+    e->chunk = (T + e->nblocks - 1) / e->nblocks;
+    e->chunk = ((e->chunk + 3) / 4) * 4;
+    e->nblocks = (int)((T + e->chunk - 1) / e->chunk);
+    const int NS = kStatsPerJ * m + kStatsPerI;
+    e->pstride = (size_t)M * NS * NP + M + 1;
+    e->pstride = (e->pstride + 31) & ~(size_t)31;
+    ALLOC(e->part, (size_t)e->nblocks * e->pstride * sizeof(double));
+    const int tiles = (e->ldb / 64) * (NP / 64);
+    e->nchunks = (int)std::max(1L, std::min((long)(2 * e->cus + tiles - 1) / tiles, (T + 31) / 32));
+    ALLOC(e->gxpart, (size_t)e->nchunks * e->ldb * NP * sizeof(double));
+    ALLOC(e->raw, e->pstride * sizeof(double));
+    ALLOC(e->gxtot, (size_t)e->ldb * NP * sizeof(double));
+    const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+    ALLOC(e->A, M * nn * sizeof(double));
+    ALLOC(e->c, M * (size_t)n * sizeof(double));
+    ALLOC(e->alpha, M * nm * sizeof(double));
+    ALLOC(e->mu, M * nm * sizeof(double));
+    ALLOC(e->beta, M * nm * sizeof(double));
+    ALLOC(e->rho, M * nm * sizeof(double));
+    ALLOC(e->gm, M * sizeof(double));
+    ALLOC(e->W, M * nn * sizeof(double));
+    ALLOC(e->ldet, M * sizeof(double));
+    ALLOC(e->lconst, M * sizeof(double));
+    ALLOC(e->Wg, (size_t)e->ldb * NP * (e->bf16 ? sizeof(uint16_t) : sizeof(float)));
+    ALLOC(e->wc, (size_t)e->ldb * sizeof(float));
+    ALLOC(e->cst, (size_t)M * kNumCst * m * NP * sizeof(float));
+    e->stride = canonical_stride(n, m);
+    ALLOC(e->stats, ((size_t)M * e->stride + 1) * sizeof(double));
+    ALLOC(e->sigma2, M * (size_t)n * sizeof(double));
+    ALLOC(e->cnew, M * (size_t)n * sizeof(double));
+    ALLOC(e->phis, M * nn * sizeof(double));
+    ALLOC(e->kappa, M * (size_t)n * sizeof(double));
+    ALLOC(e->lambda, M * (size_t)n * sizeof(double));
+    ALLOC(e->Bm, M * nn * sizeof(double));
+    ALLOC(e->work, (8 + 2 * M * nn) * sizeof(double));
+    ALLOC(e->lu, std::max((size_t)M, (size_t)2) * nn * sizeof(double));
+    ALLOC(e->bflag, M * sizeof(int));
+    ALLOC(e->newton, M * sizeof(int));
+    ALLOC(e->ds, sizeof(DevState));
+    const size_t cap = e->o.max_iter;
+    ALLOC(e->log_ll, cap * sizeof(double));
+    ALLOC(e->log_lrate, cap * sizeof(double));
+    ALLOC(e->log_vsum, cap * M * sizeof(double));
+    ALLOC(e->log_bflag, cap * M * sizeof(int));
+    ALLOC(e->log_newton, cap * M * sizeof(int));
+    ALLOC(e->log_lldec, cap * sizeof(int));
+    ALLOC(e->S, nn * sizeof(double));
+    ALLOC(e->mean, n * sizeof(double));
+    ALLOC(e->ldetS_d, sizeof(double));
+#undef ALLOC
+    *out = e;
+    return 0;
+}
+
+int amica_engine_destroy(amica_engine* e) {
+    if (!e) return 0;
+    (void)hipSetDevice(e->o.device);
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    void* ptrs[] = {e->xraw, e->xw, e->xbf, e->b, e->g, e->part, e->gxpart, e->raw, e->gxtot,
+                    e->A, e->c, e->alpha, e->mu, e->beta, e->rho, e->gm, e->W, e->ldet,
+                    e->lconst, e->Wg, e->wc, e->cst, e->stats, e->sigma2, e->cnew, e->phis,
+                    e->kappa, e->lambda, e->Bm, e->work, e->lu, e->bflag, e->newton, e->ds,
+                    e->log_ll, e->log_lrate, e->log_vsum, e->log_bflag, e->log_newton,
+                    e->log_lldec, e->ltall, e->S, e->mean, e->ldetS_d};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto& s : e->evs)
+        for (auto& v : s) (void)hipEventDestroy(v);
+    if (e->st) (void)hipStreamDestroy(e->st);
+    delete e;
+    return 0;
+}
+
+int amica_engine_cu_count(amica_engine* e) { return e ? e->cus : 0; }
+
+int amica_engine_load_raw(amica_engine* e, const float* x, int x_on_device) {
+    if (!e || !x) return fail(AMICA_E_INVALID, "null argument");
+    HIPCHK(hipSetDevice(e->o.device));
+    const size_t bytes = (size_t)e->o.T * e->o.n * sizeof(float);
+    if (!e->xraw) HIPCHK(hipMalloc((void**)&e->xraw, bytes));
+    HIPCHK(hipMemcpyAsync(e->xraw, x, bytes, x_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    e->have_data = 1;
+    return 0;
+}
+
+int amica_engine_center(amica_engine* e) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    if (!e->have_data) return fail(AMICA_E_STATE, "no raw data loaded");
+    HIPCHK(hipSetDevice(e->o.device));
+    const int n = e->o.n;
+    const long T = e->o.T;
+    const int nchunks = (int)std::max(1L, std::min((long)e->cus * 2, T / 64 + 1));
+    double* part = nullptr;
+    HIPCHK(hipMalloc((void**)&part, (size_t)nchunks * n * sizeof(double)));
+    const int rc = launch_colmean(e->xraw, T, n, part, nchunks, e->mean, e->st);
+    const hipError_t se = hipStreamSynchronize(e->st);
+    (void)hipFree(part);
+    if (rc) return fail(AMICA_E_DEVICE, "centering launch failed");
+    if (se != hipSuccess) return fail(AMICA_E_DEVICE, std::string("centering: ") + hipGetErrorString(se));
+    e->have_mean = 1;
+    return 0;
+}
+
+int amica_engine_whiten(amica_engine* e, int do_sphere) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    if (!e->have_mean) return fail(AMICA_E_STATE, "whiten before center");
+    HIPCHK(hipSetDevice(e->o.device));
+    const int n = e->o.n;
+    const long T = e->o.T;
+    const int nchunks = (int)std::max(1L, std::min((long)e->cus * 2, T / 64 + 1));
+    double *part = nullptr, *C = nullptr, *V = nullptr, *d = nullptr;
+    const size_t partn = std::max((size_t)nchunks * n * n, (size_t)nchunks * 2 * n);
+    HIPCHK(hipMalloc((void**)&part, partn * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&C, (size_t)n * n * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&V, (size_t)n * n * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&d, (size_t)n * sizeof(double)));
+    int rc = 0;
+    do {
+        if (do_sphere == 1) {
+            if (launch_cov(e->xraw, e->mean, T, n, part, nchunks, C, e->st)) { rc = -1; break; }
+            if (launch_eigh(C, V, d, n, nullptr, e->st)) { rc = -1; break; }
+        }
+        if (launch_sphere_matrix(V, d, C, n, e->xraw, e->mean, T, do_sphere, e->S, e->ldetS_d,
+                                 part, nchunks, e->st)) { rc = -1; break; }
+        if (launch_whiten(e->xraw, e->mean, e->S, T, n, e->NP, e->xw, e->xbf, e->st)) { rc = -1; break; }
+    } while (0);
+    const hipError_t se = hipStreamSynchronize(e->st);
+    (void)hipFree(part); (void)hipFree(C); (void)hipFree(V); (void)hipFree(d);
+    if (rc) return fail(AMICA_E_DEVICE, std::string("sphering launch failed: ") + hipGetErrorString(hipGetLastError()));
+    if (se != hipSuccess) return fail(AMICA_E_DEVICE, std::string("sphering: ") + hipGetErrorString(se));
+    HIPCHK(hipMemcpy(&e->ldetS, e->ldetS_d, sizeof(double), hipMemcpyDeviceToHost));
+    e->have_white = 1;
+    return 0;
+}
+
+int amica_engine_preprocess(amica_engine* e, int do_sphere) {
+    int rc = amica_engine_center(e);
+    if (rc) return rc;
+    return amica_engine_whiten(e, do_sphere);
+}
+
+int amica_engine_get_sphere(amica_engine* e, double* S, double* mean, double* ldetS) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    HIPCHK(hipSetDevice(e->o.device));
+    const int n = e->o.n;
+    if (S) HIPCHK(hipMemcpy(S, e->S, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost));
+    if (mean) HIPCHK(hipMemcpy(mean, e->mean, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    if (ldetS) *ldetS = e->ldetS;
+    return 0;
+}
+
+int amica_engine_load_white(amica_engine* e, const double* xw, double ldetS) {
+    if (!e || !xw) return fail(AMICA_E_INVALID, "null argument");
+    HIPCHK(hipSetDevice(e->o.device));
+    const size_t cnt = (size_t)e->o.T * e->o.n;
+    double* tmp = nullptr;
+    HIPCHK(hipMalloc((void**)&tmp, cnt * sizeof(double)));
+    HIPCHK(hipMemcpy(tmp, xw, cnt * sizeof(double), hipMemcpyHostToDevice));
+    int rc = launch_white_from_f64(tmp, e->o.T, e->o.n, e->NP, e->xw, e->xbf, e->st);
+    hipError_t se = hipStreamSynchronize(e->st);
+    (void)hipFree(tmp);
+    if (rc || se != hipSuccess) return fail(AMICA_E_DEVICE, "load_white failed");
+    e->ldetS = ldetS;
+    /* S and mean are unknown for externally whitened data: identity, zero */
+    std::vector<double> I((size_t)e->o.n * e->o.n, 0.0);
+    for (int i = 0; i < e->o.n; ++i) I[(size_t)i * e->o.n + i] = 1.0;
+    HIPCHK(hipMemcpy(e->S, I.data(), I.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(e->mean, 0, e->o.n * sizeof(double)));
+    e->have_white = 1;
+    return 0;
+}
+
+int amica_engine_get_white(amica_engine* e, float* x) {
+    if (!e || !x) return fail(AMICA_E_INVALID, "null argument");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipMemcpy2D(x, (size_t)e->o.n * sizeof(float), e->xw, (size_t)e->NP * sizeof(float),
+                       (size_t)e->o.n * sizeof(float), e->o.T, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int amica_engine_set_state(amica_engine* e, const double* A, const double* c, const double* alpha,
+                           const double* mu, const double* beta, const double* rho,
+                           const double* gm) {
+    if (!e || !A || !c || !alpha || !mu || !beta || !rho || !gm)
+        return fail(AMICA_E_INVALID, "null argument");
+    HIPCHK(hipSetDevice(e->o.device));
+    const int n = e->o.n, M = e->o.M, m = e->o.m;
+    const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+    HIPCHK(hipStreamSynchronize(e->st));
+    HIPCHK(hipMemcpy(e->A, A, M * nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->c, c, M * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->alpha, alpha, M * nm * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->mu, mu, M * nm * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->beta, beta, M * nm * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->rho, rho, M * nm * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->gm, gm, M * sizeof(double), hipMemcpyHostToDevice));
+    DevState ds;
+    std::memset(&ds, 0, sizeof(ds));
+    ds.lrate = e->o.lrate0;
+    ds.lratemax_local = e->o.lratemax;
+    ds.ll_prev = -HUGE_VAL;
+    ds.ldetS = e->ldetS;
+    HIPCHK(hipMemcpy(e->ds, &ds, sizeof(ds), hipMemcpyHostToDevice));
+    e->have_state = 1;
+    return 0;
+}
+
+int amica_engine_get_state(amica_engine* e, double* A, double* c, double* alpha, double* mu,
+                           double* beta, double* rho, double* gm, double* W) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    const int n = e->o.n, M = e->o.M, m = e->o.m;
+    const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+    if (A) HIPCHK(hipMemcpy(A, e->A, M * nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (c) HIPCHK(hipMemcpy(c, e->c, M * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    if (alpha) HIPCHK(hipMemcpy(alpha, e->alpha, M * nm * sizeof(double), hipMemcpyDeviceToHost));
+    if (mu) HIPCHK(hipMemcpy(mu, e->mu, M * nm * sizeof(double), hipMemcpyDeviceToHost));
+    if (beta) HIPCHK(hipMemcpy(beta, e->beta, M * nm * sizeof(double), hipMemcpyDeviceToHost));
+    if (rho) HIPCHK(hipMemcpy(rho, e->rho, M * nm * sizeof(double), hipMemcpyDeviceToHost));
+    if (gm) HIPCHK(hipMemcpy(gm, e->gm, M * sizeof(double), hipMemcpyDeviceToHost));
+    if (W) HIPCHK(hipMemcpy(W, e->W, M * nn * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int check_ready(amica_engine* e) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    if (!e->have_white) return fail(AMICA_E_STATE, "no whitened data (load_raw + preprocess, or load_white)");
+    if (!e->have_state) return fail(AMICA_E_STATE, "no model state (set_state)");
+    return 0;
+}
+
+int amica_engine_iterate(amica_engine* e, int k) {
+    int rc = check_ready(e);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->o.device));
+    for (int it = 0; it < k; ++it) {
+        if (e->timing) {
+            rc = e->next_event_set();
+            if (rc) return rc;
+        }
+        rc = e->enqueue_iteration(e->timing != 0);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int amica_engine_iterate_graph(amica_engine* e, int k) {
+    int rc = check_ready(e);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->o.device));
+    if (!e->gexec) {
+        HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+        rc = e->enqueue_iteration(false);
+        hipGraph_t gr = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->st, &gr);
+        if (rc) return rc;
+        if (ce != hipSuccess) return fail(AMICA_E_DEVICE, std::string("capture: ") + hipGetErrorString(ce));
+        e->graph = gr;
+        HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+    }
+    for (int it = 0; it < k; ++it) HIPCHK(hipGraphLaunch(e->gexec, e->st));
+    return 0;
+}
+
+int amica_engine_sync(amica_engine* e) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    return 0;
+}
+
+int amica_engine_iterations(amica_engine* e) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    DevState ds;
+    if (hipSetDevice(e->o.device) != hipSuccess) return AMICA_E_DEVICE;
+    if (hipStreamSynchronize(e->st) != hipSuccess) return AMICA_E_DEVICE;
+    if (hipMemcpy(&ds, e->ds, sizeof(ds), hipMemcpyDeviceToHost) != hipSuccess) return AMICA_E_DEVICE;
+    return ds.iter;
+}
+
+int amica_engine_get_log(amica_engine* e, int n_iter, double* ll, double* lrate, int* bflag,
+                         int* newton, int* lldec, double* vsum) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    if (n_iter < 0 || n_iter > e->o.max_iter) return fail(AMICA_E_INVALID, "n_iter exceeds the log capacity");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    const size_t M = e->o.M;
+    if (ll) HIPCHK(hipMemcpy(ll, e->log_ll, n_iter * sizeof(double), hipMemcpyDeviceToHost));
+    if (lrate) HIPCHK(hipMemcpy(lrate, e->log_lrate, n_iter * sizeof(double), hipMemcpyDeviceToHost));
+    if (bflag) HIPCHK(hipMemcpy(bflag, e->log_bflag, n_iter * M * sizeof(int), hipMemcpyDeviceToHost));
+    if (newton) HIPCHK(hipMemcpy(newton, e->log_newton, n_iter * M * sizeof(int), hipMemcpyDeviceToHost));
+    if (lldec) HIPCHK(hipMemcpy(lldec, e->log_lldec, n_iter * sizeof(int), hipMemcpyDeviceToHost));
+    if (vsum) HIPCHK(hipMemcpy(vsum, e->log_vsum, n_iter * M * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int amica_engine_estep_stats(amica_engine* e, double* stats) {
+    int rc = check_ready(e);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->o.device));
+    rc = e->enqueue_estep(false);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (stats)
+        HIPCHK(hipMemcpy(stats, e->stats, ((size_t)e->o.M * e->stride + 1) * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int amica_engine_record_ltall(amica_engine* e, int on) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    HIPCHK(hipSetDevice(e->o.device));
+    if (on && !e->ltall) {
+        HIPCHK(hipMalloc((void**)&e->ltall, (size_t)e->o.M * e->o.T * sizeof(float)));
+        if (e->gexec) {   /* the captured graph does not record ltall */
+            (void)hipGraphExecDestroy(e->gexec);
+            (void)hipGraphDestroy(e->graph);
+            e->gexec = nullptr;
+            e->graph = nullptr;
+        }
+    }
+    if (e->ltall_on != (on != 0) && e->gexec) {
+        (void)hipGraphExecDestroy(e->gexec);
+        (void)hipGraphDestroy(e->graph);
+        e->gexec = nullptr;
+        e->graph = nullptr;
+    }
+    e->ltall_on = on != 0;
+    return 0;
+}
+
+int amica_engine_get_ltall(amica_engine* e, double* lt) {
+    if (!e || !lt) return fail(AMICA_E_INVALID, "null argument");
+    if (!e->ltall) return fail(AMICA_E_STATE, "ltall recording was not enabled");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    const size_t cnt = (size_t)e->o.M * e->o.T;
+    std::vector<float> tmp(cnt);
+    HIPCHK(hipMemcpy(tmp.data(), e->ltall, cnt * sizeof(float), hipMemcpyDeviceToHost));
+    double c0 = 0.0;
+    HIPCHK(hipMemcpy(&c0, e->work, sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < cnt; ++k) lt[k] = (double)tmp[k] + c0;
+    return 0;
+}
+
+int amica_engine_output(amica_engine* e, double* A_out, double* c_out) {
+    if (!e || !A_out || !c_out) return fail(AMICA_E_INVALID, "null argument");
+    HIPCHK(hipSetDevice(e->o.device));
+    const int n = e->o.n, M = e->o.M;
+    const size_t nn = (size_t)n * n;
+    double *lu = nullptr, *Sinv = nullptr, *Ao = nullptr, *co = nullptr;
+    HIPCHK(hipMalloc((void**)&lu, 2 * nn * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&Sinv, nn * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&Ao, M * nn * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&co, (size_t)M * n * sizeof(double)));
+    int rc = launch_output_transform(e->S, e->mean, e->A, e->c, n, M, lu, Sinv, Ao, co, e->st);
+    hipError_t se = hipStreamSynchronize(e->st);
+    if (!rc && se == hipSuccess) {
+        se = hipMemcpy(A_out, Ao, M * nn * sizeof(double), hipMemcpyDeviceToHost);
+        if (se == hipSuccess) se = hipMemcpy(c_out, co, (size_t)M * n * sizeof(double), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(lu); (void)hipFree(Sinv); (void)hipFree(Ao); (void)hipFree(co);
+    if (rc || se != hipSuccess) return fail(AMICA_E_DEVICE, "output transform failed");
+    return 0;
+}
+
+int amica_engine_set_timing(amica_engine* e, int on) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    e->timing = on;
+    e->ev_used = 0;
+    return 0;
+}
+
+int amica_engine_get_timing(amica_engine* e, double* ms, int* count) {
+    if (!e || !ms) return fail(AMICA_E_INVALID, "null argument");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    double sum[4] = {0, 0, 0, 0};
+    for (size_t s = 0; s < e->ev_used; ++s) {
+        const auto& v = e->evs[s];
+        for (int k = 0; k < 4; ++k) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, v[k], v[k + 1]));
+            sum[k] += t;
+        }
+    }
+    for (int k = 0; k < 4; ++k) ms[k] = e->ev_used ? sum[k] / (double)e->ev_used : 0.0;
+    if (count) *count = (int)e->ev_used;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+/*
+ * engine_internal.h -- device-buffer layouts and kernel launch entry points
+ * of the AMICA engine (internal to libcudamica_amd).
+ */
+#ifndef AMICA_ENGINE_INTERNAL_H_
+#define AMICA_ENGINE_INTERNAL_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace amica {
+
+/* Per-(h, j, i) FP32 constants of the E-step, table [M][kNumCst][m][NP]. */
+enum {
+    kCstSb = 0,     /* sqrt(beta) */
+    kCstNsbmu = 1,  /* -sqrt(beta) mu            y = sb b + nsbmu */
+    kCstRm2 = 2,    /* rho - 2 */
+    kCstCq2 = 3,    /* (log alpha + .5 log beta - log2 - lgamma(1 + 1/rho)) log2(e) */
+    kCstRsb = 4,    /* rho sqrt(beta)             g = sum_j rsb u sign(y)|y|^(rho-1) */
+    kNumCst = 5
+};
+
+/* Raw E-step statistics per (h, j, i), in the order they are accumulated. */
+enum {
+    kRawU = 0,      /* sum u */
+    kRawUs = 1,     /* sum u sy          sy = sign(y)|y|^(rho-1) */
+    kRawUs2 = 2,    /* sum u sy^2 */
+    kRawUm2 = 3,    /* sum u |y|^(rho-2) */
+    kRawUr = 4,     /* sum u |y|^rho */
+    kRawUr2 = 5,    /* sum u |y|^(2 rho) */
+    kRawUrl = 6,    /* sum u |y|^rho log2|y| */
+    kStatsPerJ = 7
+};
+/* Raw statistics per (h, i) after the m*kStatsPerJ block. */
+enum { kRawS2 = 0, kRawCnew = 1, kRawGsum = 2, kStatsPerI = 3 };
+
+constexpr int kMaxEstepWaves = 8;   /* M * ceil(n/64) <= 8 for the fused E-step */
+
+struct EstepArgs {
+    const float* b;        /* [T][ldb] */
+    const float* x;        /* [T][NP] whitened data */
+    void* g;               /* [T][ldb] fp32 or bf16 */
+    const float* cst;      /* [M][kNumCst][m][NP] */
+    const double* lconst;  /* [M] log gm + ldet + ldetS */
+    double* part;          /* [nblocks][pstride] */
+    float* ltall;          /* optional [M][T], L_ht - max_h const (natural log) */
+    long T;
+    long chunk;            /* samples per workgroup, multiple of 4 */
+    size_t pstride;
+    int n, NP, M, CW, ldb;
+    int g_bf16;
+};
+
+int launch_estep(const EstepArgs& a, int m, int nblocks, hipStream_t st);
+
+/* GEMMs ------------------------------------------------------------------ */
+/* b[t][c] = sum_k x[t][k] W[c][k] - wc[c]    (c < ldb, k < NP) */
+int launch_unmix_f32(const float* x, const float* Wg, const float* wc, float* b,
+                     long T, int NP, int ldb, hipStream_t st);
+int launch_unmix_bf16(const uint16_t* x, const uint16_t* Wg, const float* wc, float* b,
+                      long T, int NP, int ldb, hipStream_t st);
+/* part[chunk][c][k] = sum_{t in chunk} g[t][c] x[t][k]   (FP64 partials) */
+int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, int ldb,
+                  int nchunks, hipStream_t st);
+int launch_gx_bf16(const uint16_t* g, const uint16_t* x, double* part, long T, int NP, int ldb,
+                   int nchunks, hipStream_t st);
+
+/* Reductions and the M-step ------------------------------------------------ */
+struct DevState {
+    int iter;              /* completed iterations since set_state */
+    int numdec;
+    int no_newt;
+    int pad;
+    double lrate;
+    double lratemax_local;
+    double ll_prev;
+    double ldetS;
+};
+
+struct MstepArgs {
+    int n, NP, M, m, ldb, prec_bf16;
+    long T;
+    int log_cap;
+    /* options */
+    int do_newton, update_rho, rho_start_iter, newt_start_iter;
+    int update_gm, update_alpha, update_A, update_mu, update_beta, do_reparm;
+    double lratemax, lnatrate, lratefact, rholrate, rhomin, rhomax;
+    int maxdec;
+    /* parameters (FP64) */
+    double *A, *c, *alpha, *mu, *beta, *rho, *gm, *W, *ldet, *lconst;
+    /* GEMM-precision copies */
+    void* Wg;              /* [ldb][NP] fp32 or bf16 */
+    float* wc;             /* [ldb] */
+    float* cst;            /* E-step constants */
+    /* statistics */
+    double* stats;         /* canonical, M*stride + 1 */
+    long stride;
+    /* work */
+    double *sigma2, *cnew, *phis, *kappa, *lambda, *B, *work, *lu;
+    int* bflag;            /* [M] */
+    int* newton;           /* [M] */
+    DevState* ds;
+    /* logs */
+    double* log_ll; double* log_lrate; int* log_bflag; int* log_newton; int* log_lldec;
+    double* log_vsum;
+};
+
+/* Sum E-step partials and the statistic-GEMM partials, then write the
+ * canonical statistics (including phi through the unmixing identity). */
+int launch_reduce(const double* part, int nparts, size_t pstride, const double* gxpart,
+                  int nchunks, double* raw, double* gxtot, const MstepArgs& ma, hipStream_t st);
+int launch_mstep(const MstepArgs& ma, hipStream_t st);
+/* LU inverse of A -> W, ldet, GEMM-precision W, W c, E-step constants. */
+int launch_prepare(const MstepArgs& ma, hipStream_t st);
+
+/* Preprocessing -------------------------------------------------------------- */
+int launch_colmean(const float* x, long T, int n, double* part, int nchunks, double* mean,
+                   hipStream_t st);
+int launch_cov(const float* x, const double* mean, long T, int n, double* part, int nchunks,
+               double* C, hipStream_t st);
+int launch_eigh(double* C, double* V, double* d, int n, double* work, hipStream_t st);
+int launch_sphere_matrix(const double* V, const double* d, const double* C, int n,
+                         const float* x, const double* mean, long T, int do_sphere,
+                         double* S, double* ldetS_out, double* part, int nchunks,
+                         hipStream_t st);
+int launch_whiten(const float* x, const double* mean, const double* S, long T, int n, int NP,
+                  float* xw, uint16_t* xbf, hipStream_t st);
+int launch_white_from_f64(const double* xw64, long T, int n, int NP, float* xw, uint16_t* xbf,
+                          hipStream_t st);
+int launch_bf16_from_f32(const float* x, size_t count, uint16_t* out, hipStream_t st);
+/* Sinv = S^-1 (LU), A_out = colnorm(Sinv A_h), c_out = Sinv c_h + mean. */
+int launch_output_transform(const double* S, const double* mean, const double* A,
+                            const double* c, int n, int M, double* lu, double* Sinv,
+                            double* A_out, double* c_out, hipStream_t st);
+
+}  // namespace amica
+
+#endif

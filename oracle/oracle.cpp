@@ -1,0 +1,825 @@
+/*
+ * oracle.cpp -- CPU oracle for the AMICA EM iteration (TEST INFRASTRUCTURE).
+ *
+ * A plain FP64 restatement of the reference's EM update equations, written
+ * from the reference's kernels and driver:
+ *   preprocessing    fraimondo/cudamica src/preprocess.cu:31-420
+ *   E-step           src/amica.cu:996-1611, src/kernels.cu:251-827
+ *   M-step / Newton  src/amica.cu:1963-2447, src/kernels.cu:848-1065
+ *   output transform src/amica.cu:2542-2668
+ * The per-sample formulas are written literally as the reference writes them
+ * (pow, lgamma, 1/exp(.), copysign), so that the HIP engine's algebraic
+ * rewrite of the same quantities is checked against an independent form.
+ * The reference defects listed below are resolved to the intended equation:
+ *   - y slab h*m + j (the reference indexes h*M + j, amica.cu:1462,1481,1519)
+ *   - W = pinv(A_current) (the reference may read a stale A, amica.cu:840-851)
+ *   - mean in FP64 (the reference sums float partials, kernels.cu:66)
+ *   - log|det| as a sum of log|u_ii| (reference multiplies the diagonal,
+ *     kernels.cu:190-196)
+ *   - 0*log(0) = 0 in rho_numer (kernels.cu:805-807 produce NaN at y == 0)
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this library; the product never does.
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+/* The reference's truncated log(2) constant (config.h:25). */
+const double kLog2Ref = 0.693147180559945;
+
+/* ---------------------------------------------------------------------- */
+/* Small dense FP64 linear algebra (column-major)                          */
+/* ---------------------------------------------------------------------- */
+
+/* LU with partial pivoting in place; returns sum log|u_ii| and the sign of
+ * the permutation in *sign.  piv receives the pivot rows. */
+double lu_inplace(int n, double* a, int* piv, int* sign) {
+    double lad = 0.0;
+    int s = 1;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = std::fabs(a[k + (size_t)k * n]);
+        for (int i = k + 1; i < n; ++i) {
+            double v = std::fabs(a[i + (size_t)k * n]);
+            if (v > best) { best = v; p = i; }
+        }
+        piv[k] = p;
+        if (p != k) {
+            s = -s;
+            for (int j = 0; j < n; ++j) std::swap(a[k + (size_t)j * n], a[p + (size_t)j * n]);
+        }
+        double d = a[k + (size_t)k * n];
+        lad += std::log(std::fabs(d));
+        if (d == 0.0) continue;
+        for (int i = k + 1; i < n; ++i) a[i + (size_t)k * n] /= d;
+        for (int j = k + 1; j < n; ++j) {
+            double f = a[k + (size_t)j * n];
+            if (f == 0.0) continue;
+            double* col = a + (size_t)j * n;
+            const double* lk = a + (size_t)k * n;
+            for (int i = k + 1; i < n; ++i) col[i] -= lk[i] * f;
+        }
+    }
+    if (sign) *sign = s;
+    return lad;
+}
+
+/* One-sided (Hestenes) Jacobi SVD.  On exit u holds A's columns rotated to
+ * mutual orthogonality (column j = sigma_j * left vector j) and v the right
+ * vectors. */
+void jacobi_svd(int n, const double* A, std::vector<double>& u, std::vector<double>& v) {
+    u.assign(A, A + (size_t)n * n);
+    v.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) v[i + (size_t)i * n] = 1.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        bool rotated = false;
+        for (int p = 0; p < n - 1; ++p) {
+            for (int q = p + 1; q < n; ++q) {
+                double* up = &u[(size_t)p * n];
+                double* uq = &u[(size_t)q * n];
+                double al = 0, be = 0, ga = 0;
+                for (int i = 0; i < n; ++i) {
+                    al += up[i] * up[i];
+                    be += uq[i] * uq[i];
+                    ga += up[i] * uq[i];
+                }
+                if (std::fabs(ga) <= 1e-15 * std::sqrt(al * be) || ga == 0.0) continue;
+                rotated = true;
+                double zeta = (be - al) / (2.0 * ga);
+                double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / std::sqrt(1.0 + t * t);
+                double s = c * t;
+                for (int i = 0; i < n; ++i) {
+                    double a1 = up[i], a2 = uq[i];
+                    up[i] = c * a1 - s * a2;
+                    uq[i] = s * a1 + c * a2;
+                }
+                double* vp = &v[(size_t)p * n];
+                double* vq = &v[(size_t)q * n];
+                for (int i = 0; i < n; ++i) {
+                    double a1 = vp[i], a2 = vq[i];
+                    vp[i] = c * a1 - s * a2;
+                    vq[i] = s * a1 + c * a2;
+                }
+            }
+        }
+        if (!rotated) break;
+    }
+}
+
+/* Cyclic Jacobi eigen-decomposition of a symmetric matrix; eigenvalues
+ * ascending, eigenvectors as columns. */
+void jacobi_eigh(int n, const double* C, double* evals, double* evecs) {
+    std::vector<double> a(C, C + (size_t)n * n);
+    std::vector<double> v((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) v[i + (size_t)i * n] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int q = 0; q < n; ++q)
+            for (int p = 0; p < n; ++p) {
+                double x = a[p + (size_t)q * n];
+                if (p == q) diag += x * x; else off += x * x;
+            }
+        if (off <= 1e-30 * diag || off == 0.0) break;
+        for (int p = 0; p < n - 1; ++p) {
+            for (int q = p + 1; q < n; ++q) {
+                double apq = a[p + (size_t)q * n];
+                if (apq == 0.0) continue;
+                double app = a[p + (size_t)p * n], aqq = a[q + (size_t)q * n];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                double c = 1.0 / std::sqrt(t * t + 1.0);
+                double s = t * c;
+                /* A <- J^T A J with J the (p,q) rotation */
+                for (int k = 0; k < n; ++k) {
+                    double akp = a[k + (size_t)p * n], akq = a[k + (size_t)q * n];
+                    a[k + (size_t)p * n] = c * akp - s * akq;
+                    a[k + (size_t)q * n] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = a[p + (size_t)k * n], aqk = a[q + (size_t)k * n];
+                    a[p + (size_t)k * n] = c * apk - s * aqk;
+                    a[q + (size_t)k * n] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = v[k + (size_t)p * n], vkq = v[k + (size_t)q * n];
+                    v[k + (size_t)p * n] = c * vkp - s * vkq;
+                    v[k + (size_t)q * n] = s * vkp + c * vkq;
+                }
+            }
+        }
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        return a[x + (size_t)x * n] < a[y + (size_t)y * n];
+    });
+    for (int j = 0; j < n; ++j) {
+        int o = order[j];
+        evals[j] = a[o + (size_t)o * n];
+        for (int i = 0; i < n; ++i) evecs[i + (size_t)j * n] = v[i + (size_t)o * n];
+    }
+}
+
+void pinv_impl(int n, const double* A, double* W) {
+    std::vector<double> u, v;
+    jacobi_svd(n, A, u, v);
+    std::fill(W, W + (size_t)n * n, 0.0);
+    /* W = sum_j v_j u_j^T / sigma_j^2 with u_j = sigma_j * left vector;
+     * zero singular values are truncated as amica.cu:64-76 does. */
+    for (int j = 0; j < n; ++j) {
+        const double* uj = &u[(size_t)j * n];
+        const double* vj = &v[(size_t)j * n];
+        double s2 = 0.0;
+        for (int i = 0; i < n; ++i) s2 += uj[i] * uj[i];
+        if (s2 == 0.0) continue;
+        for (int k = 0; k < n; ++k) {
+            double f = uj[k] / s2;
+            for (int i = 0; i < n; ++i) W[i + (size_t)k * n] += vj[i] * f;
+        }
+    }
+}
+
+/* C = A * B (n x n, column-major) */
+void matmul(int n, const double* A, const double* B, double* C) {
+    std::fill(C, C + (size_t)n * n, 0.0);
+    for (int k = 0; k < n; ++k)
+        for (int l = 0; l < n; ++l) {
+            double b = B[l + (size_t)k * n];
+            const double* al = A + (size_t)l * n;
+            double* ck = C + (size_t)k * n;
+            for (int i = 0; i < n; ++i) ck[i] += al[i] * b;
+        }
+}
+
+/* Digamma from the Stirling asymptotic series with Bernoulli-number
+ * coefficients B_2k/(2k), evaluated at x+6 and brought back with the
+ * recurrence psi(x) = psi(x+1) - 1/x (Bernardo 1976, AS 103; the same
+ * method as kernels.cu:896-929). */
+double psi_series(double x0) {
+    double x = x0 + 6.0;
+    double x2 = x * x;
+    double inv = 1.0 / x2;
+    static const double coef[9] = {
+        1.0 / 12.0, -1.0 / 120.0, 1.0 / 252.0, -1.0 / 240.0, 1.0 / 132.0,
+        -691.0 / 32760.0, 1.0 / 12.0, -3617.0 / 8160.0, 43867.0 / 14364.0};
+    double r = std::log(x) - 0.5 / x;
+    double p = inv;
+    for (int k = 0; k < 9; ++k) { r -= coef[k] * p; p *= inv; }
+    for (int k = 5; k >= 0; --k) r -= 1.0 / (x0 + k);
+    return r;
+}
+
+/* ---------------------------------------------------------------------- */
+/* E-step                                                                   */
+/* ---------------------------------------------------------------------- */
+
+struct Model {
+    int n, M, m;
+    int update_rho;
+    std::vector<double> W;      /* n*n*M, pinv(A_h) */
+    std::vector<double> wc;     /* n*M, W_h c_h */
+    std::vector<double> Lconst; /* M, log gm + ldet + ldetS */
+    const double *alpha, *mu, *beta, *rho;
+};
+
+
+/* Canonical statistics offsets within one model block. */
+struct SOff {
+    long usum, kn, ln, mn, md, bn, bd, rn, s2, cnew, phi, stride;
+    SOff(int n, int m) {
+        long nm = (long)n * m;
+        usum = 1; kn = usum + nm; ln = kn + nm; mn = ln + nm; md = mn + nm;
+        bn = md + nm; bd = bn + nm; rn = bd + nm; s2 = rn + nm; cnew = s2 + n;
+        phi = cnew + n; stride = phi + (long)n * n;
+    }
+};
+
+template <typename R>
+struct RMath;
+template <>
+struct RMath<double> {
+    static double pw(double a, double b) { return std::pow(a, b); }
+    static double lg(double a) { return std::log(a); }
+    static double ex(double a) { return std::exp(a); }
+    static double sq(double a) { return std::sqrt(a); }
+    static double cs(double a, double b) { return std::copysign(a, b); }
+};
+template <>
+struct RMath<float> {
+    static float pw(float a, float b) { return std::pow(a, b); }
+    static float lg(float a) { return std::log(a); }
+    static float ex(float a) { return std::exp(a); }
+    static float sq(float a) { return std::sqrt(a); }
+    static float cs(float a, float b) { return std::copysign(a, b); }
+};
+
+/* Accumulates the raw (pre-finalisation) statistics of samples [t0,t1) into
+ * acc (canonical layout; kappa/mu_denom/beta scalings are applied later). */
+template <typename R>
+void estep_range(const Model& P, const double* x, long t0, long t1,
+                 double* acc, double* llsum, double* ltall, long T) {
+    typedef RMath<R> F;
+    const int n = P.n, M = P.M, m = P.m;
+    const SOff o(n, m);
+    const long nm = (long)n * m;
+    std::vector<R> Wr((size_t)n * n * M), wcr((size_t)n * M);
+    for (size_t i = 0; i < Wr.size(); ++i) Wr[i] = (R)P.W[i];
+    for (size_t i = 0; i < wcr.size(); ++i) wcr[i] = (R)P.wc[i];
+    std::vector<R> sqb((size_t)nm * M), mur((size_t)nm * M), rhor((size_t)nm * M),
+        cq((size_t)nm * M);
+    for (long q = 0; q < nm * M; ++q) {
+        double a = P.alpha[q], be = P.beta[q], r = P.rho[q];
+        sqb[q] = (R)std::sqrt(be);
+        mur[q] = (R)P.mu[q];
+        rhor[q] = (R)r;
+        /* logalogblogpfun: log(alpha) + 0.5 log(beta) - LOG2 - lgamma(1 + 1/rho)
+         * (kernels.cu:282-307) */
+        cq[q] = (R)(std::log(a) + 0.5 * std::log(be) - kLog2Ref - std::lgamma(1.0 + 1.0 / r));
+    }
+    const int NB = 64;
+    std::vector<R> xr(n), b((size_t)n * M), y((size_t)nm * M), z((size_t)nm * M);
+    std::vector<double> L(M), v(M);
+    std::vector<double> gbuf((size_t)NB * n * M), bbuf((size_t)NB * n * M);
+    std::vector<R> g(n);
+    int nbuf = 0;
+    auto flush_phi = [&]() {
+        for (int h = 0; h < M; ++h) {
+            double* phi = acc + h * o.stride + o.phi;
+            for (int s = 0; s < nbuf; ++s) {
+                const double* gs = &gbuf[((size_t)s * M + h) * n];
+                const double* bs = &bbuf[((size_t)s * M + h) * n];
+                for (int k = 0; k < n; ++k) {
+                    double bk = bs[k];
+                    double* col = phi + (size_t)k * n;
+                    for (int i = 0; i < n; ++i) col[i] += gs[i] * bk;
+                }
+            }
+        }
+        nbuf = 0;
+    };
+    for (long t = t0; t < t1; ++t) {
+        const double* xt = x + (size_t)t * n;
+        for (int i = 0; i < n; ++i) xr[i] = (R)xt[i];
+        for (int h = 0; h < M; ++h) {
+            /* b = W x - W c (cublasDgemm + Dgemv + substract, amica.cu:1050-1102) */
+            R* bh = &b[(size_t)h * n];
+            const R* Wh = &Wr[(size_t)h * n * n];
+            for (int i = 0; i < n; ++i) bh[i] = 0;
+            for (int k = 0; k < n; ++k) {
+                R xk = xr[k];
+                const R* col = Wh + (size_t)k * n;
+                for (int i = 0; i < n; ++i) bh[i] += col[i] * xk;
+            }
+            for (int i = 0; i < n; ++i) bh[i] -= wcr[(size_t)h * n + i];
+            /* y, Q (betabyxminusmu, logalogblogpfun) and updateltz */
+            double lsum = 0.0;
+            for (int i = 0; i < n; ++i) {
+                R qv[16];
+                R qmax = 0;
+                for (int j = 0; j < m; ++j) {
+                    long q = (long)h * nm + (long)j * n + i;
+                    R yy = sqb[q] * (bh[i] - mur[q]);
+                    y[q] = yy;
+                    R Q = cq[q] - F::pw(std::fabs(yy), rhor[q]);
+                    qv[j] = Q;
+                    if (j == 0 || qmax < Q) qmax = Q;
+                }
+                R s = 0;
+                for (int j = 0; j < m; ++j) s += F::ex(qv[j] - qmax);
+                R lq = F::lg(s);
+                for (int j = 0; j < m; ++j) {
+                    long q = (long)h * nm + (long)j * n + i;
+                    z[q] = (R)1.0 / F::ex(qmax + lq - qv[j]);
+                }
+                lsum += (double)(qmax + lq);
+            }
+            L[h] = P.Lconst[h] + lsum;
+        }
+        /* updatell: P = logsumexp_h L, v = 1/exp(P - L) (kernels.cu:376-450) */
+        double Lmax = L[0];
+        for (int h = 1; h < M; ++h) Lmax = std::max(Lmax, L[h]);
+        double se = 0.0;
+        for (int h = 0; h < M; ++h) se += std::exp(L[h] - Lmax);
+        double Pt = Lmax + std::log(se);
+        for (int h = 0; h < M; ++h) v[h] = (double)((R)1.0 / F::ex((R)(Pt - L[h])));
+        *llsum += Pt;
+        if (ltall) for (int h = 0; h < M; ++h) ltall[(size_t)h * T + t] = L[h];
+        for (int h = 0; h < M; ++h) {
+            double* a = acc + h * o.stride;
+            const R vh = (R)v[h];
+            const R* bh = &b[(size_t)h * n];
+            a[0] += v[h];
+            for (int i = 0; i < n; ++i) {
+                a[o.cnew + i] += (double)(vh * xr[i]);
+                a[o.s2 + i] += (double)(vh * (bh[i] * bh[i]));
+                g[i] = 0;
+            }
+            for (int j = 0; j < m; ++j) {
+                for (int i = 0; i < n; ++i) {
+                    long q = (long)h * nm + (long)j * n + i;
+                    long k = (long)j * n + i;
+                    R yy = y[q], rr = rhor[q];
+                    R u = vh * z[q];                                      /* computeu */
+                    R fp = rr * F::cs(F::pw(std::fabs(yy), rr - (R)1.0), yy); /* computefp */
+                    R ufp = u * fp;
+                    g[i] += sqb[q] * ufp;                                 /* computeg */
+                    a[o.usum + k] += (double)u;                           /* getusum */
+                    a[o.kn + k] += (double)(ufp * fp);                    /* getdkappanumer */
+                    R lt = fp * yy - (R)1.0;
+                    a[o.ln + k] += (double)(u * (lt * lt));               /* getdlambdanumer */
+                    a[o.mn + k] += (double)ufp;                           /* getbetamu */
+                    if (rr <= (R)2.0) {
+                        a[o.md + k] += (double)(ufp / yy);
+                        a[o.bd + k] += (double)(ufp * yy);
+                    } else {
+                        a[o.md + k] += (double)(ufp * fp);
+                        a[o.bd + k] += (double)(u * F::pw(std::fabs(yy), rr));
+                    }
+                    if (P.update_rho) {                                   /* getdrhonumer */
+                        R pv = F::pw(std::fabs(yy), rr);
+                        if (pv > (R)0) a[o.rn + k] += (double)(u * (pv * F::lg(pv)));
+                    }
+                }
+            }
+            for (int i = 0; i < n; ++i) {
+                gbuf[((size_t)nbuf * M + h) * n + i] = (double)g[i];
+                bbuf[((size_t)nbuf * M + h) * n + i] = (double)bh[i];
+            }
+        }
+        if (++nbuf == NB) flush_phi();
+    }
+    if (nbuf) flush_phi();
+}
+
+/* Applies the scalings the reference applies after the per-block sums:
+ * kappa_numer = beta * sum(ufp fp), mu_denom = sqrt(beta) * sum(...),
+ * beta_denom *= rho for rho > 2, beta_numer = usum (rho <= 2) else 0
+ * (kernels.cu:583-776). */
+void finalize_stats(int n, int M, int m, const double* beta, const double* rho, double* st) {
+    const SOff o(n, m);
+    const long nm = (long)n * m;
+    for (int h = 0; h < M; ++h) {
+        double* a = st + h * o.stride;
+        for (long k = 0; k < nm; ++k) {
+            double be = beta[h * nm + k], r = rho[h * nm + k];
+            a[o.kn + k] *= be;
+            a[o.md + k] *= std::sqrt(be);
+            if (r > 2.0) a[o.bd + k] *= r;
+            a[o.bn + k] = (r <= 2.0) ? a[o.usum + k] : 0.0;
+        }
+    }
+}
+
+void build_model(Model& P, int n, int M, int m, int update_rho, double ldetS,
+                 const double* A, const double* c, const double* alpha, const double* mu,
+                 const double* beta, const double* rho, const double* gm) {
+    P.n = n; P.M = M; P.m = m; P.update_rho = update_rho;
+    P.W.assign((size_t)n * n * M, 0.0);
+    P.wc.assign((size_t)n * M, 0.0);
+    P.Lconst.assign(M, 0.0);
+    std::vector<int> piv(n);
+    std::vector<double> lu((size_t)n * n);
+    for (int h = 0; h < M; ++h) {
+        double* Wh = &P.W[(size_t)h * n * n];
+        pinv_impl(n, A + (size_t)h * n * n, Wh);
+        /* ldet from the LU of W (amica.cu:864-924) */
+        std::copy(Wh, Wh + (size_t)n * n, lu.begin());
+        double ldet = lu_inplace(n, lu.data(), piv.data(), nullptr);
+        for (int k = 0; k < n; ++k)
+            for (int i = 0; i < n; ++i) P.wc[(size_t)h * n + i] += Wh[i + (size_t)k * n] * c[(size_t)h * n + k];
+        P.Lconst[h] = std::log(gm[h]) + ldet + ldetS;
+    }
+    P.alpha = alpha; P.mu = mu; P.beta = beta; P.rho = rho;
+}
+
+template <typename R>
+void estep_all(const Model& P, const double* x, long T, double* stats, double* ltall, int nthreads) {
+    const SOff o(P.n, P.m);
+    const long stride = o.stride;
+    const long total = stride * P.M;
+    /* A fixed partition of the samples (independent of the thread count)
+     * makes the result deterministic. */
+    const int NCH = 64;
+    std::vector<double> part((size_t)NCH * (total + 1), 0.0);
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (int ch = 0; ch < NCH; ++ch) {
+        long t0 = T * ch / NCH, t1 = T * (ch + 1) / NCH;
+        double* acc = &part[(size_t)ch * (total + 1)];
+        estep_range<R>(P, x, t0, t1, acc, acc + total, ltall, T);
+    }
+    std::fill(stats, stats + total + 1, 0.0);
+    for (int ch = 0; ch < NCH; ++ch) {
+        const double* acc = &part[(size_t)ch * (total + 1)];
+        for (long k = 0; k <= total; ++k) stats[k] += acc[k];
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* M-step (amica.cu:1963-2447)                                              */
+/* ---------------------------------------------------------------------- */
+
+struct IterLog { int bflag[64]; int newton[64]; int lldec; };
+
+void mstep(const orc_config& cfg, int iter, const double* st,
+           double* A, double* c, double* alpha, double* mu, double* beta, double* rho,
+           double* gm, const std::vector<double>& W, double* lrate, double* lratemax_local,
+           int* numdec, const double* ll, IterLog* lg) {
+    const int n = cfg.n, M = cfg.M, m = cfg.m;
+    const long T = cfg.T;
+    const SOff o(n, m);
+    const long nm = (long)n * m;
+    /* lrate on likelihood decrease (amica.cu:1985-1993) */
+    lg->lldec = 0;
+    if (iter > 1 && ll[iter - 1] < ll[iter - 2]) {
+        lg->lldec = 1;
+        *lrate = *lrate * cfg.lratefact;
+        *numdec += 1;
+        if (*numdec > cfg.maxdec) {
+            *lratemax_local *= cfg.lratefact;   /* never used again (B11) */
+            *numdec = 0;
+        }
+    }
+    std::vector<double> vsum(M), sigma2((size_t)n * M), cnew((size_t)n * M), phi((size_t)n * n * M);
+    for (int h = 0; h < M; ++h) {
+        const double* a = st + h * o.stride;
+        vsum[h] = a[0];
+        if (cfg.update_gm) gm[h] = vsum[h] / (double)T;
+        for (long k = 0; k < nm; ++k)
+            if (cfg.update_alpha) alpha[h * nm + k] = a[o.usum + k] / vsum[h];
+        for (int i = 0; i < n; ++i) {
+            sigma2[(size_t)h * n + i] = a[o.s2 + i] / vsum[h];
+            cnew[(size_t)h * n + i] = a[o.cnew + i] / vsum[h];
+        }
+        for (long k = 0; k < (long)n * n; ++k) phi[(size_t)h * n * n + k] = a[o.phi + k] / vsum[h];
+    }
+    /* updatekappalambda: new alpha, old mu (kernels.cu:848-867) */
+    std::vector<double> kappa((size_t)n * M, 0.0), lambda((size_t)n * M, 0.0);
+    for (int h = 0; h < M; ++h) {
+        const double* a = st + h * o.stride;
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < n; ++i) {
+                long k = (long)j * n + i;
+                double us = a[o.usum + k];
+                double dk = a[o.kn + k] / us;
+                double al = alpha[h * nm + k];
+                double muv = mu[h * nm + k];
+                kappa[(size_t)h * n + i] += dk * al;
+                lambda[(size_t)h * n + i] += (muv * muv * dk + a[o.ln + k] / us) * al;
+            }
+    }
+    for (int h = 0; h < M; ++h) {
+        const double* a = st + h * o.stride;
+        for (long k = 0; k < nm; ++k) {
+            if (cfg.update_mu) mu[h * nm + k] += a[o.mn + k] / a[o.md + k];
+            if (cfg.update_beta) beta[h * nm + k] *= a[o.bn + k] / a[o.bd + k];
+        }
+    }
+    if (cfg.update_rho && iter >= cfg.rho_start_iter) {
+        /* updaterho (kernels.cu:937-954) */
+        for (int h = 0; h < M; ++h) {
+            const double* a = st + h * o.stride;
+            for (long k = 0; k < nm; ++k) {
+                double r = rho[h * nm + k];
+                double ps = psi_series(1.0 + 1.0 / r);
+                double step = 1.0 - (r / ps) * a[o.rn + k] / a[o.usum + k];
+                r = r + cfg.rholrate * step;
+                if (r > cfg.rhomax) r = cfg.rhomax;
+                else if (r < cfg.rhomin) r = cfg.rhomin;
+                rho[h * nm + k] = r;
+            }
+        }
+    }
+    if (!cfg.update_A) return;
+    /* Newton / natural gradient (amica.cu:2156-2279) */
+    std::vector<double> B((size_t)n * n * M);
+    int no_newt = 0;
+    for (int h = 0; h < M; ++h) {
+        const double* s2 = &sigma2[(size_t)h * n];
+        const double* ka = &kappa[(size_t)h * n];
+        const double* la = &lambda[(size_t)h * n];
+        const double* ph = &phi[(size_t)h * n * n];
+        double* Bh = &B[(size_t)h * n * n];
+        int bflag = 0;
+        for (int k = 0; k < n; ++k)
+            for (int i = 0; i < n; ++i) {
+                double D = (ka[i] * s2[i]) * (ka[k] * s2[k]);
+                if (D <= 1.0) bflag++;
+            }
+        lg->bflag[h] = bflag;
+        if (bflag > 0) no_newt = 1;
+        int newt = (bflag == 0 && cfg.do_newton == 1 && iter >= cfg.newt_start_iter);
+        lg->newton[h] = newt;
+        if (newt) {
+            for (int k = 0; k < n; ++k)
+                for (int i = 0; i < n; ++i) {
+                    double D = (ka[i] * s2[i]) * (ka[k] * s2[k]);
+                    double nom = -s2[i] * ka[k];
+                    Bh[i + (size_t)k * n] = (nom * -ph[i + (size_t)k * n] + -ph[k + (size_t)i * n]) / (D - 1.0);
+                }
+            for (int i = 0; i < n; ++i) Bh[i + (size_t)i * n] = (1.0 - ph[i + (size_t)i * n]) / la[i];
+        } else {
+            for (int k = 0; k < n; ++k)
+                for (int i = 0; i < n; ++i)
+                    Bh[i + (size_t)k * n] = (i == k ? 1.0 : 0.0) - ph[i + (size_t)k * n];
+        }
+    }
+    /* lrate growth (amica.cu:2281-2285) */
+    if (iter >= cfg.newt_start_iter && cfg.do_newton && no_newt == 0)
+        *lrate = std::min(cfg.lratemax, *lrate + std::min(0.1, *lrate));
+    else
+        *lrate = std::min(cfg.lnatrate, *lrate + std::min(0.1, *lrate));
+    std::vector<double> AB((size_t)n * n);
+    for (int h = 0; h < M; ++h) {
+        double* Ah = A + (size_t)h * n * n;
+        matmul(n, Ah, &B[(size_t)h * n * n], AB.data());
+        double f = lg->newton[h] ? *lrate : -*lrate;
+        for (long k = 0; k < (long)n * n; ++k) Ah[k] += f * AB[k];
+    }
+    if (cfg.do_reparm) {
+        /* normalizeAsave + normalizemubeta + mu shift (amica.cu:2367-2447) */
+        for (int h = 0; h < M; ++h) {
+            double* Ah = A + (size_t)h * n * n;
+            for (int k = 0; k < n; ++k) {
+                double s = 0.0;
+                for (int i = 0; i < n; ++i) s += Ah[i + (size_t)k * n] * Ah[i + (size_t)k * n];
+                double nu = std::sqrt(s);
+                for (int i = 0; i < n; ++i) Ah[i + (size_t)k * n] /= nu;
+                for (int j = 0; j < m; ++j) {
+                    long q = h * nm + (long)j * n + k;
+                    mu[q] *= nu;
+                    beta[q] = beta[q] / (nu * nu);
+                }
+            }
+        }
+        for (int h = 0; h < M; ++h) {
+            const double* Wh = &W[(size_t)h * n * n];
+            std::vector<double> dc(n), d(n, 0.0);
+            for (int i = 0; i < n; ++i) dc[i] = cnew[(size_t)h * n + i] - c[(size_t)h * n + i];
+            for (int k = 0; k < n; ++k)
+                for (int i = 0; i < n; ++i) d[i] += Wh[i + (size_t)k * n] * dc[k];
+            for (int j = 0; j < m; ++j)
+                for (int i = 0; i < n; ++i) mu[h * nm + (long)j * n + i] -= d[i];
+        }
+    }
+    /* c <- cnew (amica.cu:2456-2462) */
+    for (long k = 0; k < (long)n * M; ++k) c[k] = cnew[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_default_config(orc_config* cfg) {
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->M = 1; cfg->m = 3; cfg->maxiter = 200;
+    cfg->do_newton = 1; cfg->update_rho = 1;
+    cfg->rho_start_iter = 1; cfg->newt_start_iter = 50;
+    cfg->update_gm = cfg->update_alpha = cfg->update_A = cfg->update_mu = cfg->update_beta = 1;
+    cfg->do_reparm = 1;
+    cfg->lrate0 = 0.1; cfg->lratemax = 1.0; cfg->lnatrate = 0.1; cfg->lratefact = 0.5;
+    cfg->rholrate = 0.1; cfg->rhomin = 1.0; cfg->rhomax = 2.0;
+    cfg->maxdec = 3;
+}
+
+long orc_stats_stride(int n, int m) { return SOff(n, m).stride; }
+
+int orc_pinv(int n, const double* A, double* W) { pinv_impl(n, A, W); return 0; }
+
+double orc_logabsdet(int n, const double* A) {
+    std::vector<double> a(A, A + (size_t)n * n);
+    std::vector<int> piv(n);
+    return lu_inplace(n, a.data(), piv.data(), nullptr);
+}
+
+int orc_eigh(int n, const double* C, double* evals, double* evecs) {
+    jacobi_eigh(n, C, evals, evecs);
+    return 0;
+}
+
+double orc_psi(double x) { return psi_series(x); }
+
+int orc_preprocess(int n, long T, const float* x_raw, int do_sphere,
+                   double* x_white, double* S, double* mean, double* ldetS) {
+    /* center (preprocess.cu:31-85) with an FP64 mean */
+    for (int i = 0; i < n; ++i) mean[i] = 0.0;
+    {
+        const int NCH = 64;
+        std::vector<double> part((size_t)NCH * n, 0.0);
+#pragma omp parallel for schedule(static)
+        for (int ch = 0; ch < NCH; ++ch) {
+            long t0 = T * ch / NCH, t1 = T * (ch + 1) / NCH;
+            double* p = &part[(size_t)ch * n];
+            for (long t = t0; t < t1; ++t)
+                for (int i = 0; i < n; ++i) p[i] += (double)x_raw[(size_t)t * n + i];
+        }
+        for (int ch = 0; ch < NCH; ++ch)
+            for (int i = 0; i < n; ++i) mean[i] += part[(size_t)ch * n + i];
+        for (int i = 0; i < n; ++i) mean[i] /= (double)T;
+    }
+    std::vector<double> xc((size_t)n * T);
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < T; ++t)
+        for (int i = 0; i < n; ++i) xc[(size_t)t * n + i] = (double)x_raw[(size_t)t * n + i] - mean[i];
+    std::fill(S, S + (size_t)n * n, 0.0);
+    if (do_sphere == 1) {
+        /* C = X X^T / T (Dsyrk), eig, S = V diag(2/sqrt(d)) V^T
+         * (preprocess.cu:129-229: the dgesv solves diag(sqrt(d)/2) V^T S = V^T) */
+        const int NCH = 64;
+        std::vector<double> part((size_t)NCH * n * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int ch = 0; ch < NCH; ++ch) {
+            long t0 = T * ch / NCH, t1 = T * (ch + 1) / NCH;
+            double* p = &part[(size_t)ch * n * n];
+            for (long t = t0; t < t1; ++t) {
+                const double* xt = &xc[(size_t)t * n];
+                for (int k = 0; k < n; ++k) {
+                    double xk = xt[k];
+                    double* col = p + (size_t)k * n;
+                    for (int i = 0; i < n; ++i) col[i] += xt[i] * xk;
+                }
+            }
+        }
+        std::vector<double> C((size_t)n * n, 0.0);
+        for (int ch = 0; ch < NCH; ++ch)
+            for (size_t k = 0; k < (size_t)n * n; ++k) C[k] += part[(size_t)ch * n * n + k];
+        for (auto& v : C) v /= (double)T;
+        std::vector<double> d(n), V((size_t)n * n);
+        jacobi_eigh(n, C.data(), d.data(), V.data());
+        for (int k = 0; k < n; ++k)
+            for (int i = 0; i < n; ++i) {
+                double s = 0.0;
+                for (int j = 0; j < n; ++j) s += V[i + (size_t)j * n] * (2.0 / std::sqrt(d[j])) * V[k + (size_t)j * n];
+                S[i + (size_t)k * n] = s;
+            }
+    } else if (do_sphere == 2) {
+        /* getvariance (kernels.cu:148-161) over the centred data, with the
+         * mean of the centred data subtracted once more */
+        std::vector<double> m2(n, 0.0), mn(n, 0.0);
+        for (long t = 0; t < T; ++t)
+            for (int i = 0; i < n; ++i) mn[i] += xc[(size_t)t * n + i];
+        for (int i = 0; i < n; ++i) mn[i] /= (double)T;
+        for (long t = 0; t < T; ++t)
+            for (int i = 0; i < n; ++i) {
+                double dlt = xc[(size_t)t * n + i] - mn[i];
+                m2[i] += dlt * dlt;
+            }
+        for (int i = 0; i < n; ++i) S[i + (size_t)i * n] = 1.0 / std::sqrt(m2[i] / (double)(T - 1));
+    } else {
+        for (int i = 0; i < n; ++i) S[i + (size_t)i * n] = 1.0;
+    }
+    /* ldetS = -log|det S| (preprocess.cu:233-282, 347-377) */
+    *ldetS = -orc_logabsdet(n, S);
+    /* x <- S x (preprocess.cu:400-415) */
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < T; ++t) {
+        const double* xt = &xc[(size_t)t * n];
+        double* out = x_white + (size_t)t * n;
+        for (int i = 0; i < n; ++i) out[i] = 0.0;
+        for (int k = 0; k < n; ++k) {
+            double xk = xt[k];
+            for (int i = 0; i < n; ++i) out[i] += S[i + (size_t)k * n] * xk;
+        }
+    }
+    return 0;
+}
+
+static int estep_dispatch(int precision, int n, long T, int M, int m, int update_rho,
+                          const double* x_white, double ldetS, const double* A, const double* c,
+                          const double* alpha, const double* mu, const double* beta,
+                          const double* rho, const double* gm, double* stats, double* ltall,
+                          int nthreads) {
+    Model P;
+    build_model(P, n, M, m, update_rho, ldetS, A, c, alpha, mu, beta, rho, gm);
+    if (precision == 32) estep_all<float>(P, x_white, T, stats, ltall, nthreads);
+    else estep_all<double>(P, x_white, T, stats, ltall, nthreads);
+    finalize_stats(n, M, m, beta, rho, stats);
+    return 0;
+}
+
+int orc_estep(int n, long T, int M, int m, int update_rho, const double* x_white, double ldetS,
+              const double* A, const double* c, const double* alpha, const double* mu,
+              const double* beta, const double* rho, const double* gm, double* stats,
+              int nthreads) {
+    return estep_dispatch(64, n, T, M, m, update_rho, x_white, ldetS, A, c, alpha, mu, beta, rho,
+                          gm, stats, nullptr, nthreads);
+}
+
+int orc_estep_f32(int n, long T, int M, int m, int update_rho, const double* x_white,
+                  double ldetS, const double* A, const double* c, const double* alpha,
+                  const double* mu, const double* beta, const double* rho, const double* gm,
+                  double* stats, int nthreads) {
+    return estep_dispatch(32, n, T, M, m, update_rho, x_white, ldetS, A, c, alpha, mu, beta, rho,
+                          gm, stats, nullptr, nthreads);
+}
+
+int orc_run(const orc_config* cfgp, const double* x_white, double ldetS,
+            double* A, double* c, double* alpha, double* mu, double* beta,
+            double* rho, double* gm, double* W_last,
+            double* ll, double* lrate_log, int* bflag_log, int* newton_log,
+            int* lldec_log, double* ltall, int precision, int nthreads) {
+    const orc_config& cfg = *cfgp;
+    const int n = cfg.n, M = cfg.M, m = cfg.m;
+    if (M > 64) return -1;
+    const long stride = SOff(n, m).stride;
+    std::vector<double> st(stride * M + 1);
+    double lrate = cfg.lrate0, lratemax_local = cfg.lratemax;
+    int numdec = 0;
+    for (int iter = 1; iter <= cfg.maxiter; ++iter) {
+        Model P;
+        build_model(P, n, M, m, cfg.update_rho, ldetS, A, c, alpha, mu, beta, rho, gm);
+        if (W_last && iter == cfg.maxiter) std::copy(P.W.begin(), P.W.end(), W_last);
+        double* lt = (ltall && iter == cfg.maxiter) ? ltall : nullptr;
+        if (precision == 32) estep_all<float>(P, x_white, cfg.T, st.data(), lt, nthreads);
+        else estep_all<double>(P, x_white, cfg.T, st.data(), lt, nthreads);
+        finalize_stats(n, M, m, beta, rho, st.data());
+        ll[iter - 1] = st[stride * M] / ((double)n * (double)cfg.T);
+        if (lrate_log) lrate_log[iter - 1] = lrate;
+        IterLog lg;
+        std::memset(&lg, 0, sizeof(lg));
+        mstep(cfg, iter, st.data(), A, c, alpha, mu, beta, rho, gm, P.W, &lrate, &lratemax_local,
+              &numdec, ll, &lg);
+        for (int h = 0; h < M; ++h) {
+            if (bflag_log) bflag_log[(size_t)(iter - 1) * M + h] = lg.bflag[h];
+            if (newton_log) newton_log[(size_t)(iter - 1) * M + h] = lg.newton[h];
+        }
+        if (lldec_log) lldec_log[iter - 1] = lg.lldec;
+    }
+    return 0;
+}
+
+int orc_output_transform(int n, int M, const double* S, const double* mean, double* A, double* c) {
+    std::vector<double> Sp((size_t)n * n), tmp((size_t)n * n);
+    pinv_impl(n, S, Sp.data());
+    for (int h = 0; h < M; ++h) {
+        double* ch = c + (size_t)h * n;
+        std::vector<double> r(mean, mean + n);
+        for (int k = 0; k < n; ++k)
+            for (int i = 0; i < n; ++i) r[i] += Sp[i + (size_t)k * n] * ch[k];
+        std::copy(r.begin(), r.end(), ch);
+        double* Ah = A + (size_t)h * n * n;
+        matmul(n, Sp.data(), Ah, tmp.data());
+        for (int k = 0; k < n; ++k) {
+            double s = 0.0;
+            for (int i = 0; i < n; ++i) s += tmp[i + (size_t)k * n] * tmp[i + (size_t)k * n];
+            double nu = std::sqrt(s);
+            for (int i = 0; i < n; ++i) Ah[i + (size_t)k * n] = tmp[i + (size_t)k * n] / nu;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

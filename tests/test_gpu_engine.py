@@ -1,0 +1,200 @@
+"""GPU parity of the HIP engine against the CPU oracle (run on an MI355X).
+
+Bars (FP32 path): E-step statistics within 2e-5 (relative Frobenius norm
+per statistic) and the log-likelihood sum within 1e-7 of the FP64 oracle on
+identical float32 inputs; the LL trajectory of a full run within 1e-6
+relative, identical branch sequences (Newton / natural gradient, LL
+decrease) and the final W within an Amari distance of 1e-4.  BF16 path:
+final LL within 1e-4 relative of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import cudamica_amd as ca
+from oracle import pyoracle as po
+from tests.conftest import amari_distance
+
+pytestmark = pytest.mark.gpu
+
+STAT_NAMES = ("usum", "kappa_numer", "lambda_numer", "mu_numer", "mu_denom", "beta_numer",
+              "beta_denom", "rho_numer", "sigma2_numer", "cnew", "phi")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _random_state(n, M, m, seed):
+    st = ca.synth_init(n, M, m, seed)
+    rng = np.random.default_rng(seed)
+    st["rho"] = rng.uniform(1.0, 2.0, size=st["rho"].shape)
+    st["mu"] = st["mu"] + rng.normal(scale=0.2, size=st["mu"].shape)
+    st["beta"] = rng.uniform(0.6, 1.6, size=st["beta"].shape)
+    a = rng.uniform(0.2, 1.0, size=(M, m, n))
+    st["alpha"] = (a / a.sum(axis=1, keepdims=True)).ravel()
+    A = st["A"].reshape(M, n, n)
+    A += rng.normal(scale=0.05, size=A.shape)
+    st["A"] = A.ravel()
+    g = rng.uniform(0.5, 1.0, size=M)
+    st["gm"] = g / g.sum()
+    return st
+
+
+def _compare_stats(gpu, ora, n, m, M, tol=2e-5):
+    gv, gll = po.stats_views(gpu, n, m, M)
+    ov, oll = po.stats_views(ora, n, m, M)
+    worst = {}
+    for h in range(M):
+        assert abs(gv[h]["vsum"] - ov[h]["vsum"]) <= 1e-6 * abs(ov[h]["vsum"]), (h, gv[h]["vsum"], ov[h]["vsum"])
+        for k in STAT_NAMES:
+            a, b = np.asarray(gv[h][k]), np.asarray(ov[h][k])
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+            worst[k] = max(worst.get(k, 0.0), rel)
+    _say(f"  worst relative errors: { {k: f'{v:.2e}' for k, v in worst.items()} }")
+    for k, v in worst.items():
+        assert v < tol, f"{k}: relative error {v:.3e} >= {tol}"
+    assert abs(gll - oll) <= 1e-7 * abs(oll), (gll, oll)
+
+
+@pytest.mark.parametrize("n,T,M,m", [
+    (16, 4000, 1, 3), (64, 20000, 1, 3), (71, 6000, 2, 2), (32, 5000, 1, 1),
+    (128, 20000, 4, 3), (256, 8000, 2, 4)])
+def test_estep_stats_fp32(built, n, T, M, m):
+    _say(f"[estep fp32] n={n} T={T} M={M} m={m}")
+    spec = ca.synth_spec(n, T, seed=100 + n + M, cond_max=0.0)
+    x = ca.synth_data(spec)
+    xw, S, mean, ldetS = po.preprocess(x, 1)
+    x32 = xw.astype(np.float32).astype(np.float64)
+    st = _random_state(n, M, m, seed=7 + n)
+    ora = po.estep(x32, ldetS, st["A"], st["c"], st["alpha"], st["mu"], st["beta"], st["rho"],
+                   st["gm"], M, m)
+    with ca.Engine(ca.options(n, T, M, m, precision=ca.PREC_FP32)) as eng:
+        eng.load_white(x32, ldetS)
+        eng.set_state(st)
+        gpu = eng.estep_stats()
+    _compare_stats(gpu, ora, n, m, M)
+
+
+@pytest.mark.parametrize("sph", [1, 2, 0])
+def test_preprocess_parity(built, sph):
+    n, T = 48, 30000
+    _say(f"[preprocess] do_sphere={sph}")
+    x = ca.synth_data(ca.synth_spec(n, T, seed=55, cond_max=0.0))
+    x = x + np.linspace(-3, 3, n, dtype=np.float32)[None]     # non-zero channel means
+    xw, S, mean, ldetS = po.preprocess(x, sph)
+    with ca.Engine(ca.options(n, T, 1, 3)) as eng:
+        eng.load_raw(x)
+        eng.preprocess(sph)
+        Sg, mg, lg = eng.sphere()
+        wg = eng.white()
+    np.testing.assert_allclose(mg, mean, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(Sg, S, rtol=1e-9, atol=1e-11 * np.abs(S).max())
+    assert abs(lg - ldetS) < 1e-9 * max(1.0, abs(ldetS))
+    assert np.max(np.abs(wg - xw)) <= 2e-6 * np.max(np.abs(xw))
+
+
+def _run_both(w, T, iters, precision, newt=None, graph=False):
+    n, M, m = w.n, w.M, w.m
+    x = ca.synth_data(w.spec(T=T))
+    st0 = w.init_state()
+    xw, S, mean, ldetS = po.preprocess(x, w.do_sphere)
+    nsi = w.newt_start_iter if newt is None else newt
+    cfg = po.default_config(n, T, M, m, maxiter=iters, newt_start_iter=nsi)
+    fin, logs = po.run(cfg, xw, ldetS, st0)
+    with ca.Engine(ca.options(n, T, M, m, precision=precision, max_iter=iters,
+                              newt_start_iter=nsi)) as eng:
+        eng.load_raw(x)
+        eng.preprocess(w.do_sphere)
+        eng.set_state(st0)
+        eng.iterate(iters, graph=graph)
+        eng.sync()
+        glog = eng.log(iters)
+        gst = eng.get_state()
+    return fin, logs, gst, glog
+
+
+def _check_trajectory(fin, logs, gst, glog, n, M, ll_tol, w_tol):
+    rel = np.abs(glog["ll"] - logs["ll"]) / np.abs(logs["ll"])
+    _say(f"  LL max rel err {rel.max():.3e}; final LL gpu {glog['ll'][-1]:.12f} oracle {logs['ll'][-1]:.12f}")
+    np.testing.assert_array_equal(glog["newton"], logs["newton"])
+    np.testing.assert_array_equal(glog["bflag"] > 0, logs["bflag"] > 0)
+    np.testing.assert_array_equal(glog["lldec"], logs["lldec"])
+    np.testing.assert_allclose(glog["lrate"], logs["lrate"], rtol=1e-12)
+    assert rel.max() <= ll_tol
+    for h in range(M):
+        Wg = gst["W"][h * n * n:(h + 1) * n * n].reshape(n, n).T
+        Wo = fin["W"][h * n * n:(h + 1) * n * n].reshape(n, n).T
+        d = amari_distance(Wg, np.linalg.inv(Wo))
+        _say(f"  model {h}: Amari(W_gpu, W_oracle) = {d:.3e}")
+        assert d <= w_tol
+
+
+def test_full_run_fp32_n16(built):
+    w = ca.WORKLOADS["n16_T20k"]
+    _say("[run fp32] n16_T20k 50 iterations")
+    fin, logs, gst, glog = _run_both(w, w.T, 50, ca.PREC_FP32)
+    _check_trajectory(fin, logs, gst, glog, w.n, w.M, 1e-6, 1e-4)
+
+
+def test_full_run_fp32_newton(built):
+    w = ca.WORKLOADS["n64_T1M"]
+    _say("[run fp32] n64 subsample T=60000, 30 iterations, Newton from 10")
+    fin, logs, gst, glog = _run_both(w, 60000, 30, ca.PREC_FP32, newt=10)
+    assert glog["newton"].sum() > 0
+    _check_trajectory(fin, logs, gst, glog, w.n, w.M, 1e-6, 1e-4)
+
+
+def test_full_run_bf16_models(built):
+    w = ca.WORKLOADS["n128_T2M_M4"]
+    T = 100000
+    _say(f"[run bf16] n128 M4 subsample T={T}, 8 iterations")
+    fin, logs, gst, glog = _run_both(w, T, 8, ca.PREC_BF16)
+    rel = np.abs(glog["ll"] - logs["ll"]) / np.abs(logs["ll"])
+    _say(f"  bf16 LL rel err per iteration: {np.array2string(rel, precision=2)}")
+    assert rel[-1] <= 1e-4
+
+
+def test_graph_replay_matches_eager(built):
+    n, T, M, m = 32, 12000, 2, 3
+    _say("[graph] replay vs eager")
+    x = ca.synth_data(ca.synth_spec(n, T, seed=77, cond_max=0.0))
+    st0 = ca.synth_init(n, M, m, 78)
+    res = []
+    for graph in (False, True):
+        with ca.Engine(ca.options(n, T, M, m, max_iter=6, newt_start_iter=3)) as eng:
+            eng.load_raw(x)
+            eng.preprocess(1)
+            eng.set_state(st0)
+            for _ in range(3):       # replays separated by host synchronisation
+                eng.iterate(2, graph=graph)
+                eng.sync()
+                eng.estep_stats() if not graph else None
+            res.append((eng.log(6), eng.get_state()))
+    (la, sa), (lb, sb) = res
+    np.testing.assert_array_equal(la["ll"], lb["ll"])
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+
+
+def test_ltall_and_output_transform(built):
+    w = ca.WORKLOADS["n16_T20k"]
+    n, M, m, T = w.n, w.M, w.m, 8000
+    _say("[ltall/output] n16 T=8000 3 iterations")
+    x = ca.synth_data(w.spec(T=T))
+    st0 = w.init_state()
+    xw, S, mean, ldetS = po.preprocess(x, 1)
+    cfg = po.default_config(n, T, M, m, maxiter=3)
+    fin, logs = po.run(cfg, xw, ldetS, st0, want_ltall=True)
+    Ao, co = po.output_transform(S, mean, fin["A"], fin["c"], n, M)
+    with ca.Engine(ca.options(n, T, M, m, max_iter=3)) as eng:
+        eng.load_raw(x)
+        eng.preprocess(1)
+        eng.set_state(st0)
+        eng.iterate(2)
+        eng.record_ltall(True)
+        eng.iterate(1)
+        lt = eng.ltall()
+        Ag, cg = eng.output()
+    np.testing.assert_allclose(lt, logs["ltall"], rtol=2e-6, atol=2e-6 * np.abs(logs["ltall"]).max())
+    np.testing.assert_allclose(Ag, Ao, rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(cg, co, rtol=1e-4, atol=1e-5)
