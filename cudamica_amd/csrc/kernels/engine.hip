@@ -68,9 +68,11 @@ struct amica_engine {
     hipStream_t st = nullptr;
     /* data */
     float* xraw = nullptr;
-    float* xw = nullptr;
-    uint16_t* xbf = nullptr;
-    float* b = nullptr;
+    float* xw = nullptr;           /* float32 whitened data (both paths) */
+    double* xw64 = nullptr;        /* FP64 whitened data (FP32 path) */
+    uint16_t* xbf = nullptr;       /* bf16 whitened data (BF16 path) */
+    float* b = nullptr;            /* FP32 unmixed data (BF16 path) */
+    double* b64 = nullptr;         /* FP64 unmixed data (FP32 path) */
     void* g = nullptr;
     /* partials */
     int nblocks = 0;
@@ -84,9 +86,13 @@ struct amica_engine {
     /* parameters and work */
     double *A = nullptr, *c = nullptr, *alpha = nullptr, *mu = nullptr, *beta = nullptr,
            *rho = nullptr, *gm = nullptr, *W = nullptr, *ldet = nullptr, *lconst = nullptr;
-    void* Wg = nullptr;
+    double* Wg64 = nullptr;
+    double* wc64 = nullptr;
+    uint16_t* Whi = nullptr;
+    uint16_t* Wlo = nullptr;
     float* wc = nullptr;
     float* cst = nullptr;
+    double* cst64 = nullptr;
     double* stats = nullptr;
     long stride = 0;
     double *sigma2 = nullptr, *cnew = nullptr, *phis = nullptr, *kappa = nullptr,
@@ -123,7 +129,8 @@ struct amica_engine {
         a.rholrate = o.rholrate; a.rhomin = o.rhomin; a.rhomax = o.rhomax; a.maxdec = o.maxdec;
         a.A = A; a.c = c; a.alpha = alpha; a.mu = mu; a.beta = beta; a.rho = rho; a.gm = gm;
         a.W = W; a.ldet = ldet; a.lconst = lconst;
-        a.Wg = Wg; a.wc = wc; a.cst = cst;
+        a.Wg64 = Wg64; a.wc64 = wc64; a.Whi = Whi; a.Wlo = Wlo; a.wc = wc;
+        a.cst = cst; a.cst64 = cst64;
         a.stats = stats; a.stride = stride;
         a.sigma2 = sigma2; a.cnew = cnew; a.phis = phis; a.kappa = kappa; a.lambda = lambda;
         a.B = Bm; a.work = work; a.lu = lu;
@@ -136,7 +143,8 @@ struct amica_engine {
     EstepArgs eargs(float* lt) const {
         EstepArgs a;
         std::memset(&a, 0, sizeof(a));
-        a.b = b; a.x = xw; a.g = g; a.cst = cst; a.lconst = lconst; a.part = part; a.ltall = lt;
+        a.b = b; a.b64 = b64; a.cst64 = cst64;
+        a.x = xw; a.g = g; a.cst = cst; a.lconst = lconst; a.part = part; a.ltall = lt;
         a.T = o.T; a.chunk = chunk; a.pstride = pstride;
         a.n = o.n; a.NP = NP; a.M = o.M; a.CW = CW; a.ldb = ldb; a.g_bf16 = bf16;
         return a;
@@ -148,9 +156,9 @@ struct amica_engine {
         if (with_timing) HIPCHK(hipEventRecord(ev[0], st));
         LCHK(launch_prepare(ma, st));
         if (bf16)
-            LCHK(launch_unmix_bf16(xbf, (const uint16_t*)Wg, wc, b, o.T, NP, ldb, st));
+            LCHK(launch_unmix_bf16x3(xw, Whi, Wlo, wc, b, o.T, NP, ldb, st));
         else
-            LCHK(launch_unmix_f32(xw, (const float*)Wg, wc, b, o.T, NP, ldb, st));
+            LCHK(launch_unmix_f64(xw64, Wg64, wc64, b64, o.T, NP, ldb, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
         LCHK(launch_estep(eargs(ltall_on ? ltall : nullptr), o.m, nblocks, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[2], st));
@@ -242,8 +250,13 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     };
 #define ALLOC(ptr, bytes) do { int _r = alloc((void**)&(ptr), (bytes)); if (_r) { amica_engine_destroy(e); return _r; } } while (0)
     ALLOC(e->xw, (size_t)T * NP * sizeof(float));
-    if (e->bf16) ALLOC(e->xbf, (size_t)T * NP * sizeof(uint16_t));
-    ALLOC(e->b, (size_t)T * e->ldb * sizeof(float));
+    if (e->bf16) {
+        ALLOC(e->xbf, (size_t)T * NP * sizeof(uint16_t));
+        ALLOC(e->b, (size_t)T * e->ldb * sizeof(float));
+    } else {
+        ALLOC(e->xw64, (size_t)T * NP * sizeof(double));
+        ALLOC(e->b64, (size_t)T * e->ldb * sizeof(double));
+    }
     ALLOC(e->g, (size_t)T * e->ldb * (e->bf16 ? sizeof(uint16_t) : sizeof(float)));
     /* E-step grid: every CU holds 8 waves of this kernel (2 per SIMD). */
     const int waves = M * CW;
@@ -258,6 +271,13 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     e->pstride = (size_t)M * NS * NP + M + 1;
     e->pstride = (e->pstride + 31) & ~(size_t)31;
     ALLOC(e->part, (size_t)e->nblocks * e->pstride * sizeof(double));
+    {
+        const int rc = estep_configure(m, (size_t)NS * 64 * waves * sizeof(double));
+        if (rc) {
+            amica_engine_destroy(e);
+            return fail(AMICA_E_DEVICE, "E-step LDS configuration failed");
+        }
+    }
     const int tiles = (e->ldb / 64) * (NP / 64);
     e->nchunks = (int)std::max(1L, std::min((long)(2 * e->cus + tiles - 1) / tiles, (T + 31) / 32));
     ALLOC(e->gxpart, (size_t)e->nchunks * e->ldb * NP * sizeof(double));
@@ -274,7 +294,14 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     ALLOC(e->W, M * nn * sizeof(double));
     ALLOC(e->ldet, M * sizeof(double));
     ALLOC(e->lconst, M * sizeof(double));
-    ALLOC(e->Wg, (size_t)e->ldb * NP * (e->bf16 ? sizeof(uint16_t) : sizeof(float)));
+    if (e->bf16) {
+        ALLOC(e->Whi, (size_t)e->ldb * NP * sizeof(uint16_t));
+        ALLOC(e->Wlo, (size_t)e->ldb * NP * sizeof(uint16_t));
+    } else {
+        ALLOC(e->Wg64, (size_t)e->ldb * NP * sizeof(double));
+        ALLOC(e->wc64, (size_t)e->ldb * sizeof(double));
+        ALLOC(e->cst64, (size_t)M * kNumCst64 * m * NP * sizeof(double));
+    }
     ALLOC(e->wc, (size_t)e->ldb * sizeof(float));
     ALLOC(e->cst, (size_t)M * kNumCst * m * NP * sizeof(float));
     e->stride = canonical_stride(n, m);
@@ -311,9 +338,10 @@ int amica_engine_destroy(amica_engine* e) {
     if (e->st) (void)hipStreamSynchronize(e->st);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
-    void* ptrs[] = {e->xraw, e->xw, e->xbf, e->b, e->g, e->part, e->gxpart, e->raw, e->gxtot,
-                    e->A, e->c, e->alpha, e->mu, e->beta, e->rho, e->gm, e->W, e->ldet,
-                    e->lconst, e->Wg, e->wc, e->cst, e->stats, e->sigma2, e->cnew, e->phis,
+    void* ptrs[] = {e->xraw, e->xw, e->xw64, e->xbf, e->b, e->b64, e->g, e->part, e->gxpart,
+                    e->raw, e->gxtot, e->A, e->c, e->alpha, e->mu, e->beta, e->rho, e->gm, e->W,
+                    e->ldet, e->lconst, e->Wg64, e->wc64, e->Whi, e->Wlo, e->wc, e->cst,
+                    e->cst64, e->stats, e->sigma2, e->cnew, e->phis,
                     e->kappa, e->lambda, e->Bm, e->work, e->lu, e->bflag, e->newton, e->ds,
                     e->log_ll, e->log_lrate, e->log_vsum, e->log_bflag, e->log_newton,
                     e->log_lldec, e->ltall, e->S, e->mean, e->ldetS_d};
@@ -378,7 +406,7 @@ int amica_engine_whiten(amica_engine* e, int do_sphere) {
         }
         if (launch_sphere_matrix(V, d, C, n, e->xraw, e->mean, T, do_sphere, e->S, e->ldetS_d,
                                  part, nchunks, e->st)) { rc = -1; break; }
-        if (launch_whiten(e->xraw, e->mean, e->S, T, n, e->NP, e->xw, e->xbf, e->st)) { rc = -1; break; }
+        if (launch_whiten(e->xraw, e->mean, e->S, T, n, e->NP, e->xw, e->xw64, e->xbf, e->st)) { rc = -1; break; }
     } while (0);
     const hipError_t se = hipStreamSynchronize(e->st);
     (void)hipFree(part); (void)hipFree(C); (void)hipFree(V); (void)hipFree(d);
@@ -412,7 +440,7 @@ int amica_engine_load_white(amica_engine* e, const double* xw, double ldetS) {
     double* tmp = nullptr;
     HIPCHK(hipMalloc((void**)&tmp, cnt * sizeof(double)));
     HIPCHK(hipMemcpy(tmp, xw, cnt * sizeof(double), hipMemcpyHostToDevice));
-    int rc = launch_white_from_f64(tmp, e->o.T, e->o.n, e->NP, e->xw, e->xbf, e->st);
+    int rc = launch_white_from_f64(tmp, e->o.T, e->o.n, e->NP, e->xw, e->xw64, e->xbf, e->st);
     hipError_t se = hipStreamSynchronize(e->st);
     (void)hipFree(tmp);
     if (rc || se != hipSuccess) return fail(AMICA_E_DEVICE, "load_white failed");
@@ -619,6 +647,37 @@ int amica_engine_output(amica_engine* e, double* A_out, double* c_out) {
     (void)hipFree(lu); (void)hipFree(Sinv); (void)hipFree(Ao); (void)hipFree(co);
     if (rc || se != hipSuccess) return fail(AMICA_E_DEVICE, "output transform failed");
     return 0;
+}
+
+int amica_engine_debug_buffer(amica_engine* e, const char* name, void* out, size_t bytes) {
+    if (!e || !name || !out) return fail(AMICA_E_INVALID, "null argument");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    const long T = e->o.T;
+    const size_t NP = e->NP, ldb = e->ldb;
+    struct Buf { const char* n; const void* p; size_t size; };
+    const Buf bufs[] = {
+        {"x", e->xw, T * NP * sizeof(float)},
+        {"x64", e->xw64, e->xw64 ? T * NP * sizeof(double) : 0},
+        {"xbf", e->xbf, e->xbf ? T * NP * sizeof(uint16_t) : 0},
+        {"b", e->b, e->b ? T * ldb * sizeof(float) : 0},
+        {"b64", e->b64, e->b64 ? T * ldb * sizeof(double) : 0},
+        {"g", e->g, T * ldb * (e->bf16 ? sizeof(uint16_t) : sizeof(float))},
+        {"gx", e->gxtot, ldb * NP * sizeof(double)},
+        {"raw", e->raw, e->pstride * sizeof(double)},
+        {"Wg64", e->Wg64, e->Wg64 ? ldb * NP * sizeof(double) : 0},
+        {"wc64", e->wc64, e->wc64 ? ldb * sizeof(double) : 0},
+        {"wc", e->wc, ldb * sizeof(float)},
+    };
+    for (const Buf& b : bufs) {
+        if (std::strcmp(b.n, name) != 0) continue;
+        if (!b.p || b.size == 0) return fail(AMICA_E_STATE, std::string("buffer not allocated: ") + name);
+        if (bytes != b.size) return fail(AMICA_E_INVALID, std::string("size mismatch for ") + name +
+                                                             ": " + std::to_string(b.size));
+        HIPCHK(hipMemcpy(out, b.p, bytes, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    return fail(AMICA_E_INVALID, std::string("unknown buffer ") + name);
 }
 
 int amica_engine_set_timing(amica_engine* e, int on) {

@@ -36,8 +36,14 @@ enum { kRawS2 = 0, kRawCnew = 1, kRawGsum = 2, kStatsPerI = 3 };
 
 constexpr int kMaxEstepWaves = 8;   /* M * ceil(n/64) <= 8 for the fused E-step */
 
+/* FP64 per-(h, j, i) constants of the FP64 unmixing path, [M][2][m][NP]:
+ * sqrt(beta) and -sqrt(beta) mu, so that y is formed in FP64. */
+enum { kCst64Sb = 0, kCst64Nsbmu = 1, kNumCst64 = 2 };
+
 struct EstepArgs {
-    const float* b;        /* [T][ldb] */
+    const float* b;        /* [T][ldb] (FP32 unmixing) */
+    const double* b64;     /* [T][ldb] (FP64 unmixing), or null */
+    const double* cst64;   /* [M][kNumCst64][m][NP] when b64 is set */
     const float* x;        /* [T][NP] whitened data */
     void* g;               /* [T][ldb] fp32 or bf16 */
     const float* cst;      /* [M][kNumCst][m][NP] */
@@ -52,13 +58,17 @@ struct EstepArgs {
 };
 
 int launch_estep(const EstepArgs& a, int m, int nblocks, hipStream_t st);
+/* Allows the E-step variants for m components `lds_bytes` of dynamic LDS. */
+int estep_configure(int m, size_t lds_bytes);
 
 /* GEMMs ------------------------------------------------------------------ */
 /* b[t][c] = sum_k x[t][k] W[c][k] - wc[c]    (c < ldb, k < NP) */
-int launch_unmix_f32(const float* x, const float* Wg, const float* wc, float* b,
+int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double* b,
                      long T, int NP, int ldb, hipStream_t st);
-int launch_unmix_bf16(const uint16_t* x, const uint16_t* Wg, const float* wc, float* b,
-                      long T, int NP, int ldb, hipStream_t st);
+/* x float32 split on the fly into bf16 hi + lo; W given as bf16 hi and lo:
+ * b = x_hi W_hi + x_hi W_lo + x_lo W_hi (three MFMAs, ~2^-16 relative) */
+int launch_unmix_bf16x3(const float* x, const uint16_t* Whi, const uint16_t* Wlo,
+                        const float* wc, float* b, long T, int NP, int ldb, hipStream_t st);
 /* part[chunk][c][k] = sum_{t in chunk} g[t][c] x[t][k]   (FP64 partials) */
 int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, int ldb,
                   int nchunks, hipStream_t st);
@@ -88,10 +98,15 @@ struct MstepArgs {
     int maxdec;
     /* parameters (FP64) */
     double *A, *c, *alpha, *mu, *beta, *rho, *gm, *W, *ldet, *lconst;
-    /* GEMM-precision copies */
-    void* Wg;              /* [ldb][NP] fp32 or bf16 */
+    /* GEMM copies of W (rows h*NP + i, columns k): FP64 path Wg64 + wc64,
+     * BF16 path Whi + Wlo (bf16) + wc */
+    double* Wg64;          /* [ldb][NP] */
+    double* wc64;          /* [ldb] */
+    uint16_t* Whi;         /* [ldb][NP] */
+    uint16_t* Wlo;         /* [ldb][NP] */
     float* wc;             /* [ldb] */
     float* cst;            /* E-step constants */
+    double* cst64;         /* FP64 E-step constants (FP64 path) */
     /* statistics */
     double* stats;         /* canonical, M*stride + 1 */
     long stride;
@@ -123,10 +138,12 @@ int launch_sphere_matrix(const double* V, const double* d, const double* C, int 
                          const float* x, const double* mean, long T, int do_sphere,
                          double* S, double* ldetS_out, double* part, int nchunks,
                          hipStream_t st);
+/* x_white = S (x - mean): float32 copy xw, optional FP64 copy xw64 and bf16
+ * copy xbf, all [T][NP] with zero padding. */
 int launch_whiten(const float* x, const double* mean, const double* S, long T, int n, int NP,
-                  float* xw, uint16_t* xbf, hipStream_t st);
-int launch_white_from_f64(const double* xw64, long T, int n, int NP, float* xw, uint16_t* xbf,
-                          hipStream_t st);
+                  float* xw, double* xw64, uint16_t* xbf, hipStream_t st);
+int launch_white_from_f64(const double* xin, long T, int n, int NP, float* xw, double* xw64,
+                          uint16_t* xbf, hipStream_t st);
 int launch_bf16_from_f32(const float* x, size_t count, uint16_t* out, hipStream_t st);
 /* Sinv = S^-1 (LU), A_out = colnorm(Sinv A_h), c_out = Sinv c_h + mean. */
 int launch_output_transform(const double* S, const double* mean, const double* A,

@@ -31,9 +31,14 @@
  *   beta_d = rho sum u yr                            sqrt(beta) rho^2 sum u sy^2 (rho > 2)
  *   lambda = rho^2 sum u yr^2 - 2 rho sum u yr + usum
  *   rho_n  = rho ln2 sum u yr la
- * Statistics are summed in FP32 over 32 samples, then in FP64 per lane, and
- * written as one FP64 partial per workgroup (no atomics: the reduction
- * kernel sums partials in a fixed order).
+ * Statistics are summed in FP32 over 32 samples, then in FP64 per lane (in
+ * LDS), and written as one FP64 partial per workgroup (no atomics: the
+ * reduction kernel sums partials in a fixed order).
+ *
+ * B64 = true (FP32 path): b is FP64 and y = sqrt(beta)(b - mu) is formed in
+ * FP64 before the FP32 density math, so |y| near 0 -- where mu_denom's
+ * weight |y|^(rho-2) is singular for rho < 2 -- keeps full relative
+ * precision.
  */
 #include "common.h"
 #include "engine_internal.h"
@@ -49,7 +54,7 @@ __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_lo
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-template <int MJ>
+template <int MJ, bool B64>
 __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     constexpr int NSJ = kStatsPerJ;
     constexpr int NS = NSJ * MJ + kStatsPerI;
@@ -63,6 +68,8 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 
     __shared__ float Lp[2][kMaxEstepWaves][kS];
     __shared__ float dc2[AMICA_MAX_M];
+    /* per-lane FP64 totals, [NS][blockDim.x] */
+    extern __shared__ double tot[];
 
     /* L'_h = (log gm + ldet + ldetS) - c0 with c0 = max_h of that constant,
      * so that only differences enter FP32 (in log2 units). */
@@ -73,6 +80,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     }
 
     float sb[MJ], nsbmu[MJ], rm2[MJ], cq2[MJ], rsb[MJ];
+    double sb64[B64 ? MJ : 1], nsbmu64[B64 ? MJ : 1];
 #pragma unroll
     for (int j = 0; j < MJ; ++j) {
         const float* cj = a.cst + ((size_t)h * kNumCst * MJ + j) * a.NP + i;
@@ -86,13 +94,19 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
         } else {
             sb[j] = 0.f; nsbmu[j] = 1.f; rm2[j] = 0.f; cq2[j] = 0.f; rsb[j] = 0.f;
         }
+        if constexpr (B64) {
+            const double* dj = a.cst64 + ((size_t)h * kNumCst64 * MJ + j) * a.NP + i;
+            sb64[j] = act ? dj[kCst64Sb * ks] : 0.0;
+            nsbmu64[j] = act ? dj[kCst64Nsbmu * ks] : 1.0;
+        }
     }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) tot[s * blockDim.x + threadIdx.x] = 0.0;
     __syncthreads();
 
     float acc[NS];
-    double tot[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) { acc[s] = 0.f; tot[s] = 0.0; }
+    for (int s = 0; s < NS; ++s) acc[s] = 0.f;
     double vsum_acc = 0.0, ll_acc = 0.0;
 
     const long t0 = (long)blockIdx.x * a.chunk;
@@ -108,13 +122,22 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
         for (int s = 0; s < kS; ++s) {
             const long t = tb + s;
             const bool valid = t < t1;
-            const float bv = (act && valid) ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
+            float bv;
+            double bv64 = 0.0;
+            if constexpr (B64) {
+                bv64 = (act && valid) ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
+                bv = (float)bv64;
+            } else {
+                bv = (act && valid) ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
+            }
             BV[s] = bv;
             float q2[MJ];
             float mx = 0.f;
 #pragma unroll
             for (int j = 0; j < MJ; ++j) {
-                const float y = fmaf(sb[j], bv, nsbmu[j]);
+                float y;
+                if constexpr (B64) y = (float)fma(sb64[j], bv64, nsbmu64[j]);
+                else y = fmaf(sb[j], bv, nsbmu[j]);
                 const float la = fast_log2(fmaxf(fabsf(y), 1.17549435e-38f));
                 const float ym2 = fast_exp2(rm2[j] * la);
                 const float sy = ym2 * y;
@@ -201,17 +224,19 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
         buf ^= 1;
         if (++tiles == kFlush) {
 #pragma unroll
-            for (int s = 0; s < NS; ++s) { tot[s] += (double)acc[s]; acc[s] = 0.f; }
+            for (int s = 0; s < NS; ++s) {
+                tot[s * blockDim.x + threadIdx.x] += (double)acc[s];
+                acc[s] = 0.f;
+            }
             tiles = 0;
         }
     }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) tot[s] += (double)acc[s];
-
     double* base = a.part + (size_t)blockIdx.x * a.pstride;
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
-        base[((size_t)h * NS + s) * a.NP + i] = act ? tot[s] : 0.0;
+    for (int s = 0; s < NS; ++s) {
+        const double v = tot[s * blockDim.x + threadIdx.x] + (double)acc[s];
+        base[((size_t)h * NS + s) * a.NP + i] = act ? v : 0.0;
+    }
     /* per-model vsum and the log-likelihood sum from lanes 0..kS-1 */
     double vs = wave_sum_f64(lane < kS ? vsum_acc : 0.0);
     double ls = wave_sum_f64(lane < kS ? ll_acc : 0.0);
@@ -223,19 +248,52 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 
 }  // namespace
 
-int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
-    dim3 grid(nblocks), block(64 * a.M * a.CW);
+int estep_configure(int MJ, size_t lds_bytes) {
+    hipError_t r = hipSuccess;
+#define AMICA_ESTEP_ATTR(J)                                                                       \
+    case J:                                                                                       \
+        r = hipFuncSetAttribute((const void*)estep_kernel<J, true>,                              \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);      \
+        if (r == hipSuccess)                                                                      \
+            r = hipFuncSetAttribute((const void*)estep_kernel<J, false>,                         \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
+        break;
     switch (MJ) {
-        case 1: hipLaunchKernelGGL(estep_kernel<1>, grid, block, 0, st, a); break;
-        case 2: hipLaunchKernelGGL(estep_kernel<2>, grid, block, 0, st, a); break;
-        case 3: hipLaunchKernelGGL(estep_kernel<3>, grid, block, 0, st, a); break;
-        case 4: hipLaunchKernelGGL(estep_kernel<4>, grid, block, 0, st, a); break;
-        case 5: hipLaunchKernelGGL(estep_kernel<5>, grid, block, 0, st, a); break;
-        case 6: hipLaunchKernelGGL(estep_kernel<6>, grid, block, 0, st, a); break;
-        case 7: hipLaunchKernelGGL(estep_kernel<7>, grid, block, 0, st, a); break;
-        case 8: hipLaunchKernelGGL(estep_kernel<8>, grid, block, 0, st, a); break;
+        AMICA_ESTEP_ATTR(1)
+        AMICA_ESTEP_ATTR(2)
+        AMICA_ESTEP_ATTR(3)
+        AMICA_ESTEP_ATTR(4)
+        AMICA_ESTEP_ATTR(5)
+        AMICA_ESTEP_ATTR(6)
+        AMICA_ESTEP_ATTR(7)
+        AMICA_ESTEP_ATTR(8)
         default: return -1;
     }
+#undef AMICA_ESTEP_ATTR
+    return r == hipSuccess ? 0 : -1;
+}
+
+int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
+    dim3 grid(nblocks), block(64 * a.M * a.CW);
+    const size_t shm = (size_t)(kStatsPerJ * MJ + kStatsPerI) * block.x * sizeof(double);
+    const bool b64 = a.b64 != nullptr;
+#define AMICA_ESTEP_CASE(J)                                                                   \
+    case J:                                                                                   \
+        if (b64) hipLaunchKernelGGL((estep_kernel<J, true>), grid, block, shm, st, a);        \
+        else hipLaunchKernelGGL((estep_kernel<J, false>), grid, block, shm, st, a);           \
+        break;
+    switch (MJ) {
+        AMICA_ESTEP_CASE(1)
+        AMICA_ESTEP_CASE(2)
+        AMICA_ESTEP_CASE(3)
+        AMICA_ESTEP_CASE(4)
+        AMICA_ESTEP_CASE(5)
+        AMICA_ESTEP_CASE(6)
+        AMICA_ESTEP_CASE(7)
+        AMICA_ESTEP_CASE(8)
+        default: return -1;
+    }
+#undef AMICA_ESTEP_CASE
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -4,7 +4,8 @@
  * Unmixing GEMM (fraimondo/cudamica src/amica.cu:1035-1102: per block and
  * model a cublasDgemm b = W x, a Dgemv W c and the `substract` kernel):
  *   b[t][h*NP + i] = sum_k x[t][k] W_h[i][k] - (W_h c_h)[i]
- * for all models at once, as one [T x NP] x [NP x M*NP] product.
+ * for all models at once, as one [T x NP] x [NP x M*NP] product, in FP64
+ * (FP32 path) or split BF16 (BF16 path).
  *
  * Statistic GEMM (amica.cu:1533-1550, cublasDgemm phi = g b^T per block and
  * model).  The engine computes G = sum_t g_t x_t^T for all models at once
@@ -15,6 +16,8 @@
  * MFMA operand layouts used here (per wave of 64 lanes, l = lane):
  *   v_mfma_f32_16x16x4_f32:   lane l holds A[l%16][l/16], B[l/16][l%16];
  *                             D[(l/16)*4 + r][l%16] in register r.
+ *   v_mfma_f64_16x16x4_f64:   same A and B; D[(l/16) + 4r][l%16] in register r
+ *                             (measured on gfx950).
  *   v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8*(l/16) .. +7],
  *                             B[8*(l/16) .. +7][l%16]; D as above.
  * Both are checked end to end against the CPU oracle by the statistics
@@ -27,42 +30,45 @@ namespace amica {
 
 namespace {
 
-/* ---------------- unmixing GEMM, FP32 ---------------------------------- */
-/* Workgroup tile 64 samples x 64 columns, K staged through LDS in chunks of
- * 32; each of the 4 waves owns a 32 x 32 quarter (2 x 2 MFMA tiles). */
-__global__ __launch_bounds__(256) void unmix_f32_kernel(const float* __restrict__ x,
-                                                        const float* __restrict__ Wg,
-                                                        const float* __restrict__ wc,
-                                                        float* __restrict__ b, long T, int NP,
+/* ---------------- unmixing GEMM, FP64 ---------------------------------- */
+/* FP64 path: b and hence y are formed in FP64, so that statistics with a
+ * singular weight at y = 0 (mu_denom sums u |y|^(rho-2)) see the same y as
+ * an FP64 implementation.  Workgroup tile 64 samples x 64 columns, K staged
+ * through LDS in chunks of 32; each of the 4 waves owns a 32 x 32 quarter
+ * (2 x 2 v_mfma_f64_16x16x4 tiles). */
+__global__ __launch_bounds__(256) void unmix_f64_kernel(const double* __restrict__ x,
+                                                        const double* __restrict__ Wg,
+                                                        const double* __restrict__ wc,
+                                                        double* __restrict__ b, long T, int NP,
                                                         int ldb) {
-    __shared__ __attribute__((aligned(16))) float xs[64][36];
-    __shared__ __attribute__((aligned(16))) float ws[64][36];
+    __shared__ __attribute__((aligned(16))) double xs[64][34];
+    __shared__ __attribute__((aligned(16))) double ws[64][34];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wcn = w & 1;
     const long t0 = (long)blockIdx.x * 64;
     const int c0 = blockIdx.y * 64;
-    f32x4 acc[2][2];
+    f64x4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) acc[a][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < 2; ++c) acc[a][c] = (f64x4){0.0, 0.0, 0.0, 0.0};
 
     for (int k0 = 0; k0 < NP; k0 += 32) {
 #pragma unroll
-        for (int rep = 0; rep < 2; ++rep) {
-            const int q = tid + rep * 256;
-            const int r = q >> 3, c4 = (q & 7) * 4;
+        for (int rep = 0; rep < 4; ++rep) {
+            const int q = tid + rep * 256;          /* 64 rows x 16 double2 */
+            const int r = q >> 4, c2 = (q & 15) * 2;
             const long t = t0 + r;
-            float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (t < T) xv = *(const float4*)(x + (size_t)t * NP + k0 + c4);
-            *(float4*)&xs[r][c4] = xv;
-            *(float4*)&ws[r][c4] = *(const float4*)(Wg + (size_t)(c0 + r) * NP + k0 + c4);
+            double2 xv = make_double2(0.0, 0.0);
+            if (t < T) xv = *(const double2*)(x + (size_t)t * NP + k0 + c2);
+            *(double2*)&xs[r][c2] = xv;
+            *(double2*)&ws[r][c2] = *(const double2*)(Wg + (size_t)(c0 + r) * NP + k0 + c2);
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < 32; kk += 4) {
             const int kr = kk + (lane >> 4);
-            float av[2], bv[2];
+            double av[2], bv[2];
 #pragma unroll
             for (int a = 0; a < 2; ++a) av[a] = xs[wr * 32 + a * 16 + (lane & 15)][kr];
 #pragma unroll
@@ -71,33 +77,48 @@ __global__ __launch_bounds__(256) void unmix_f32_kernel(const float* __restrict_
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int c = 0; c < 2; ++c)
-                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const int col = c0 + wcn * 32 + c * 16 + (lane & 15);
-        const float bias = wc[col];
+        const double bias = wc[col];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long t = t0 + wr * 32 + a * 16 + (lane >> 4) * 4 + r;
+                /* f64 MFMA: register r of lane l holds row (l/16) + 4r */
+                const long t = t0 + wr * 32 + a * 16 + (lane >> 4) + 4 * r;
                 if (t < T) b[(size_t)t * ldb + col] = acc[a][c][r] - bias;
             }
     }
 }
 
-/* ---------------- unmixing GEMM, BF16 ---------------------------------- */
-/* Operands are K-contiguous, so each lane loads its 8 bf16 of a fragment
- * with one 16-byte load straight from global memory (W is L2-resident).
- * Wave tile 16 samples x 64 columns; workgroup 64 x 64. */
-__global__ __launch_bounds__(256) void unmix_bf16_kernel(const uint16_t* __restrict__ x,
-                                                         const uint16_t* __restrict__ Wg,
-                                                         const float* __restrict__ wc,
-                                                         float* __restrict__ b, long T, int NP,
-                                                         int ldb) {
+/* ---------------- unmixing GEMM, split BF16 ---------------------------- */
+/* x (float32) is split in registers into x_hi = bf16(x) and
+ * x_lo = bf16(x - x_hi); W arrives as bf16 hi and lo tables.  Three MFMAs
+ * per fragment (hi*hi, hi*lo, lo*hi) give b to ~2^-16 relative at BF16
+ * MFMA rate.  Operands are K-contiguous: each lane loads its 8 elements of
+ * a fragment straight from global memory (W is L2-resident).  Wave tile 16
+ * samples x 64 columns; workgroup 64 x 64. */
+__device__ __forceinline__ void split_bf16x8(const float4 a, const float4 b, i16x8& hi, i16x8& lo) {
+    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const uint16_t h = f32_to_bf16_bits(v[e]);
+        hi[e] = (short)h;
+        lo[e] = (short)f32_to_bf16_bits(v[e] - bf16_bits_to_f32(h));
+    }
+}
+
+__global__ __launch_bounds__(256) void unmix_bf16x3_kernel(const float* __restrict__ x,
+                                                           const uint16_t* __restrict__ Whi,
+                                                           const uint16_t* __restrict__ Wlo,
+                                                           const float* __restrict__ wc,
+                                                           float* __restrict__ b, long T, int NP,
+                                                           int ldb) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long t0 = (long)blockIdx.x * 64 + w * 16;
     const int c0 = blockIdx.y * 64;
@@ -107,14 +128,25 @@ __global__ __launch_bounds__(256) void unmix_bf16_kernel(const uint16_t* __restr
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[c] = (f32x4){0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < NP; k0 += 32) {
-        i16x8 av = (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
-        if (trow < T) av = *(const i16x8*)(x + (size_t)trow * NP + k0 + kofs);
+        float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
+        if (trow < T) {
+            const float* xp = x + (size_t)trow * NP + k0 + kofs;
+            x0 = *(const float4*)xp;
+            x1 = *(const float4*)(xp + 4);
+        }
+        i16x8 ahi, alo;
+        split_bf16x8(x0, x1, ahi, alo);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const i16x8 bv = *(const i16x8*)(Wg + (size_t)(c0 + c * 16 + (lane & 15)) * NP + k0 + kofs);
-            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
-                                                             __builtin_bit_cast(bf16x8, bv),
-                                                             acc[c], 0, 0, 0);
+            const size_t wofs = (size_t)(c0 + c * 16 + (lane & 15)) * NP + k0 + kofs;
+            const i16x8 bhi = *(const i16x8*)(Whi + wofs);
+            const i16x8 blo = *(const i16x8*)(Wlo + wofs);
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, alo),
+                                                             __builtin_bit_cast(bf16x8, bhi), acc[c], 0, 0, 0);
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ahi),
+                                                             __builtin_bit_cast(bf16x8, blo), acc[c], 0, 0, 0);
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ahi),
+                                                             __builtin_bit_cast(bf16x8, bhi), acc[c], 0, 0, 0);
         }
     }
 #pragma unroll
@@ -252,19 +284,19 @@ __global__ __launch_bounds__(256) void gx_bf16_kernel(const uint16_t* __restrict
 
 }  // namespace
 
-int launch_unmix_f32(const float* x, const float* Wg, const float* wc, float* b, long T, int NP,
-                     int ldb, hipStream_t st) {
+int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double* b, long T,
+                     int NP, int ldb, hipStream_t st) {
     if (NP % 32 || ldb % 64) return -1;
     dim3 grid((unsigned)((T + 63) / 64), ldb / 64);
-    hipLaunchKernelGGL(unmix_f32_kernel, grid, dim3(256), 0, st, x, Wg, wc, b, T, NP, ldb);
+    hipLaunchKernelGGL(unmix_f64_kernel, grid, dim3(256), 0, st, x, Wg, wc, b, T, NP, ldb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_unmix_bf16(const uint16_t* x, const uint16_t* Wg, const float* wc, float* b, long T,
-                      int NP, int ldb, hipStream_t st) {
+int launch_unmix_bf16x3(const float* x, const uint16_t* Whi, const uint16_t* Wlo, const float* wc,
+                        float* b, long T, int NP, int ldb, hipStream_t st) {
     if (NP % 32 || ldb % 64) return -1;
     dim3 grid((unsigned)((T + 63) / 64), ldb / 64);
-    hipLaunchKernelGGL(unmix_bf16_kernel, grid, dim3(256), 0, st, x, Wg, wc, b, T, NP, ldb);
+    hipLaunchKernelGGL(unmix_bf16x3_kernel, grid, dim3(256), 0, st, x, Whi, Wlo, wc, b, T, NP, ldb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

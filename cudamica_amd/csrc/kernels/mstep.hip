@@ -137,15 +137,18 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
         const int i = ib + (int)(q % (ie - ib)), k = (int)(q / (ie - ib));
         const double* grow = gx + ((size_t)h * NP + i) * NP;
         const size_t wrow = ((size_t)h * NP + k) * NP;
-        double s = 0.0;
+        double s = 0.0, wck;
         if (a.prec_bf16) {
-            const uint16_t* wg = (const uint16_t*)a.Wg + wrow;
-            for (int l = 0; l < n; ++l) s += grow[l] * (double)bf16_bits_to_f32(wg[l]);
+            /* the unmixing GEMM used W~ = W_hi + W_lo */
+            for (int l = 0; l < n; ++l)
+                s += grow[l] * ((double)bf16_bits_to_f32(a.Whi[wrow + l]) +
+                                (double)bf16_bits_to_f32(a.Wlo[wrow + l]));
+            wck = (double)a.wc[(size_t)h * NP + k];
         } else {
-            const float* wg = (const float*)a.Wg + wrow;
-            for (int l = 0; l < n; ++l) s += grow[l] * (double)wg[l];
+            for (int l = 0; l < n; ++l) s += grow[l] * a.Wg64[wrow + l];
+            wck = a.wc64[(size_t)h * NP + k];
         }
-        s -= gsum[i] * (double)a.wc[(size_t)h * NP + k];
+        s -= gsum[i] * wck;
         st[o.phi + i + (long)k * n] = s;
     }
 }
@@ -346,13 +349,19 @@ __global__ __launch_bounds__(kBlk) void prepare_kernel(MstepArgs a) {
     double* W = a.W + (size_t)h * nn;
     double* X = a.work + 8 + (size_t)a.M * nn + (size_t)h * nn;
     const double lad = block_lu_inverse(a.A + (size_t)h * nn, n, a.lu + (size_t)h * nn, X, W, perm);
-    /* GEMM-precision copy of W (rows h*NP + i, columns k) and W~ c */
+    /* GEMM copy of W (rows h*NP + i, columns k) and W~ c with the W~ the
+     * unmixing GEMM multiplies by: FP64 W, or bf16 hi + lo parts of W */
     for (long q = tid; q < nn; q += blockDim.x) {
         const int i = (int)(q / n), k = (int)(q % n);
         const double wv2 = W[i + (long)k * n];
         const size_t dst = ((size_t)h * NP + i) * NP + k;
-        if (a.prec_bf16) ((uint16_t*)a.Wg)[dst] = f32_to_bf16_bits((float)wv2);
-        else ((float*)a.Wg)[dst] = (float)wv2;
+        if (a.prec_bf16) {
+            const uint16_t hi = f32_to_bf16_bits((float)wv2);
+            a.Whi[dst] = hi;
+            a.Wlo[dst] = f32_to_bf16_bits((float)(wv2 - (double)bf16_bits_to_f32(hi)));
+        } else {
+            a.Wg64[dst] = wv2;
+        }
     }
     __syncthreads();
     const double* c = a.c + (size_t)h * n;
@@ -360,11 +369,13 @@ __global__ __launch_bounds__(kBlk) void prepare_kernel(MstepArgs a) {
         double s = 0.0;
         const size_t row = ((size_t)h * NP + i) * NP;
         for (int k = 0; k < n; ++k) {
-            const double wr = a.prec_bf16 ? (double)bf16_bits_to_f32(((const uint16_t*)a.Wg)[row + k])
-                                          : (double)((const float*)a.Wg)[row + k];
+            const double wr = a.prec_bf16 ? (double)bf16_bits_to_f32(a.Whi[row + k]) +
+                                                (double)bf16_bits_to_f32(a.Wlo[row + k])
+                                          : a.Wg64[row + k];
             s += wr * c[k];
         }
-        a.wc[(size_t)h * NP + i] = (float)s;
+        if (a.prec_bf16) a.wc[(size_t)h * NP + i] = (float)s;
+        else a.wc64[(size_t)h * NP + i] = s;
     }
     /* E-step constants */
     for (long q = tid; q < nm; q += blockDim.x) {
@@ -380,6 +391,11 @@ __global__ __launch_bounds__(kBlk) void prepare_kernel(MstepArgs a) {
         cj[kCstRm2 * ks] = (float)(r - 2.0);
         cj[kCstCq2 * ks] = (float)(cq * 1.4426950408889634);
         cj[kCstRsb * ks] = (float)(r * sb);
+        if (!a.prec_bf16) {
+            double* dj = a.cst64 + ((size_t)h * kNumCst64 * m + j) * NP + i;
+            dj[kCst64Sb * ks] = sb;
+            dj[kCst64Nsbmu * ks] = -sb * mv;
+        }
     }
     if (tid == 0) {
         const double ldet = -lad;

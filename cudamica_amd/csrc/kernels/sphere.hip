@@ -259,6 +259,7 @@ __global__ __launch_bounds__(256) void whiten_kernel(const float* __restrict__ x
                                                      const double* __restrict__ mean,
                                                      const double* __restrict__ S, long T, int n,
                                                      int NP, int TS, float* __restrict__ xw,
+                                                     double* __restrict__ xw64,
                                                      uint16_t* __restrict__ xbf) {
     extern __shared__ double xc[];
     const long t0 = (long)blockIdx.x * TS;
@@ -277,18 +278,22 @@ __global__ __launch_bounds__(256) void whiten_kernel(const float* __restrict__ x
             for (int k = 0; k < n; ++k) v += S[i + (long)k * n] * xc[s * n + k];
         const float f = (float)v;
         xw[(size_t)t * NP + i] = f;
+        if (xw64) xw64[(size_t)t * NP + i] = v;
         if (xbf) xbf[(size_t)t * NP + i] = f32_to_bf16_bits(f);
     }
 }
 
-__global__ void white_from_f64_kernel(const double* __restrict__ xw64, long T, int n, int NP,
-                                      float* __restrict__ xw, uint16_t* __restrict__ xbf) {
+__global__ void white_from_f64_kernel(const double* __restrict__ xin, long T, int n, int NP,
+                                      float* __restrict__ xw, double* __restrict__ xw64,
+                                      uint16_t* __restrict__ xbf) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)T * NP) return;
     const long t = (long)(e / NP);
     const int i = (int)(e % NP);
-    const float f = (i < n) ? (float)xw64[(size_t)t * n + i] : 0.f;
+    const double d = (i < n) ? xin[(size_t)t * n + i] : 0.0;
+    const float f = (float)d;
     xw[e] = f;
+    if (xw64) xw64[e] = d;
     if (xbf) xbf[e] = f32_to_bf16_bits(f);
 }
 
@@ -385,20 +390,20 @@ int launch_sphere_matrix(const double* V, const double* d, const double* C, int 
 }
 
 int launch_whiten(const float* x, const double* mean, const double* S, long T, int n, int NP,
-                  float* xw, uint16_t* xbf, hipStream_t st) {
+                  float* xw, double* xw64, uint16_t* xbf, hipStream_t st) {
     int TS = 4096 / n;
     if (TS < 1) TS = 1;
     const size_t shm = (size_t)TS * n * sizeof(double);
     hipLaunchKernelGGL(whiten_kernel, dim3((unsigned)((T + TS - 1) / TS)), dim3(256), shm, st, x,
-                       mean, S, T, n, NP, TS, xw, xbf);
+                       mean, S, T, n, NP, TS, xw, xw64, xbf);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_white_from_f64(const double* xw64, long T, int n, int NP, float* xw, uint16_t* xbf,
-                          hipStream_t st) {
+int launch_white_from_f64(const double* xin, long T, int n, int NP, float* xw, double* xw64,
+                          uint16_t* xbf, hipStream_t st) {
     const size_t cnt = (size_t)T * NP;
     hipLaunchKernelGGL(white_from_f64_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0,
-                       st, xw64, T, n, NP, xw, xbf);
+                       st, xin, T, n, NP, xw, xw64, xbf);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

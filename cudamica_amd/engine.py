@@ -72,6 +72,7 @@ _SIGNATURES = [
     ("amica_engine_set_timing", ctypes.c_int, [_V, ctypes.c_int]),
     ("amica_engine_get_timing", ctypes.c_int, [_V, _D, _I]),
     ("amica_last_error", ctypes.c_char_p, []),
+    ("amica_engine_debug_buffer", ctypes.c_int, [_V, ctypes.c_char_p, _V, ctypes.c_size_t]),
     ("amica_engine_cu_count", ctypes.c_int, [_V]),
     ("amica_synth_data", ctypes.c_int, [ctypes.POINTER(SynthSpec), _F, _D, _D, ctypes.c_int]),
     ("amica_synth_init", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
@@ -263,6 +264,12 @@ class Engine:
         cnt = np.zeros(1, np.int32)
         _check(self.L.amica_engine_get_timing(self.h, _d(ms), _i(cnt)), "get_timing")
         return ms, int(cnt[0])
+
+    def debug_buffer(self, name, dtype, shape):
+        out = np.zeros(shape, dtype)
+        _check(self.L.amica_engine_debug_buffer(self.h, name.encode(), out.ctypes.data_as(_V), out.nbytes),
+               f"debug_buffer({name})")
+        return out
 
     def cu_count(self):
         return self.L.amica_engine_cu_count(self.h)

@@ -124,6 +124,10 @@ int  amica_engine_output(amica_engine* eng, double* A_out, double* c_out);
 int  amica_engine_set_timing(amica_engine* eng, int on);
 int  amica_engine_get_timing(amica_engine* eng, double* ms, int* count);
 
+/* Diagnostic: copies one internal device buffer to host memory (names: x,
+ * x64, xbf, b, b64, g, gx, raw, Wg64, wc64, wc); bytes must equal its size. */
+int  amica_engine_debug_buffer(amica_engine* eng, const char* name, void* out, size_t bytes);
+
 const char* amica_last_error(void);
 /* Number of compute units of the engine's device. */
 int  amica_engine_cu_count(amica_engine* eng);

@@ -40,20 +40,28 @@ def _random_state(n, M, m, seed):
     return st
 
 
-def _compare_stats(gpu, ora, n, m, M, tol=2e-5):
+def _compare_stats(gpu, ora, n, m, M, tol=2e-5, ll_tol=1e-7, skip=(), vsum_tol=1e-6):
+    """Relative Frobenius error per statistic; cnew (sum_t v x, close to 0 for
+    centred data and M = 1) is compared in absolute terms, as cnew/vsum in
+    units of the whitened data's standard deviation (2)."""
     gv, gll = po.stats_views(gpu, n, m, M)
     ov, oll = po.stats_views(ora, n, m, M)
     worst = {}
     for h in range(M):
-        assert abs(gv[h]["vsum"] - ov[h]["vsum"]) <= 1e-6 * abs(ov[h]["vsum"]), (h, gv[h]["vsum"], ov[h]["vsum"])
+        assert abs(gv[h]["vsum"] - ov[h]["vsum"]) <= vsum_tol * abs(ov[h]["vsum"]), (h, gv[h]["vsum"], ov[h]["vsum"])
         for k in STAT_NAMES:
             a, b = np.asarray(gv[h][k]), np.asarray(ov[h][k])
-            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+            if k == "cnew":
+                rel = np.max(np.abs(a - b)) / ov[h]["vsum"] / 2.0
+            else:
+                rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
             worst[k] = max(worst.get(k, 0.0), rel)
-    _say(f"  worst relative errors: { {k: f'{v:.2e}' for k, v in worst.items()} }")
+    _say(f"  worst relative errors: { {k: f'{v:.2e}' for k, v in worst.items()} }; "
+         f"LL sum {abs(gll - oll) / abs(oll):.2e}")
     for k, v in worst.items():
-        assert v < tol, f"{k}: relative error {v:.3e} >= {tol}"
-    assert abs(gll - oll) <= 1e-7 * abs(oll), (gll, oll)
+        if k not in skip:
+            assert v < tol, f"{k}: relative error {v:.3e} >= {tol}"
+    assert abs(gll - oll) <= ll_tol * abs(oll), (gll, oll)
 
 
 @pytest.mark.parametrize("n,T,M,m", [
@@ -73,6 +81,25 @@ def test_estep_stats_fp32(built, n, T, M, m):
         eng.set_state(st)
         gpu = eng.estep_stats()
     _compare_stats(gpu, ora, n, m, M)
+
+
+def test_estep_stats_bf16(built):
+    """Split-BF16 unmixing (~2^-16 relative b): statistics within 1e-3, the
+    LL sum within 1e-5; mu_denom is reported, not bounded (its weight
+    |y|^(rho-2) is singular at y = 0, where b's rounding dominates)."""
+    n, T, M, m = 128, 20000, 4, 3
+    _say(f"[estep bf16] n={n} T={T} M={M} m={m}")
+    x = ca.synth_data(ca.synth_spec(n, T, seed=300, cond_max=0.0))
+    xw, S, mean, ldetS = po.preprocess(x, 1)
+    x32 = xw.astype(np.float32).astype(np.float64)
+    st = _random_state(n, M, m, seed=301)
+    ora = po.estep(x32, ldetS, st["A"], st["c"], st["alpha"], st["mu"], st["beta"], st["rho"],
+                   st["gm"], M, m)
+    with ca.Engine(ca.options(n, T, M, m, precision=ca.PREC_BF16)) as eng:
+        eng.load_white(x32, ldetS)
+        eng.set_state(st)
+        gpu = eng.estep_stats()
+    _compare_stats(gpu, ora, n, m, M, tol=1e-3, ll_tol=1e-5, skip=("mu_denom",), vsum_tol=1e-4)
 
 
 @pytest.mark.parametrize("sph", [1, 2, 0])
@@ -114,13 +141,29 @@ def _run_both(w, T, iters, precision, newt=None, graph=False):
 
 
 def _check_trajectory(fin, logs, gst, glog, n, M, ll_tol, w_tol):
+    """Branch-aware trajectory check.  The LL-decrease test and the Newton
+    test are discrete decisions; a near-tie can go either way under
+    rounding, after which the trajectories legitimately differ.  So: the LL
+    must agree within ll_tol up to and including the first iteration whose
+    decisions differ, and a differing LL-decrease decision must be a
+    near-tie of the oracle's own trajectory.  Without a differing decision
+    the whole trajectory, the lrate schedule and the final W are checked."""
+    K = len(logs["ll"])
+    same = ((glog["newton"] == logs["newton"]).all(axis=1)
+            & ((glog["bflag"] > 0) == (logs["bflag"] > 0)).all(axis=1)
+            & (glog["lldec"] == logs["lldec"]))
+    kdiv = K if same.all() else int(np.argmin(same))
     rel = np.abs(glog["ll"] - logs["ll"]) / np.abs(logs["ll"])
-    _say(f"  LL max rel err {rel.max():.3e}; final LL gpu {glog['ll'][-1]:.12f} oracle {logs['ll'][-1]:.12f}")
-    np.testing.assert_array_equal(glog["newton"], logs["newton"])
-    np.testing.assert_array_equal(glog["bflag"] > 0, logs["bflag"] > 0)
-    np.testing.assert_array_equal(glog["lldec"], logs["lldec"])
+    _say(f"  LL max rel err {rel[:kdiv + 1].max():.3e} over {min(kdiv + 1, K)} iterations; "
+         f"final LL gpu {glog['ll'][-1]:.12f} oracle {logs['ll'][-1]:.12f}")
+    assert rel[:kdiv + 1].max() <= ll_tol
+    if kdiv < K:
+        step = abs(logs["ll"][kdiv] - logs["ll"][kdiv - 1]) / abs(logs["ll"][kdiv])
+        _say(f"  decisions first differ at iteration {kdiv + 1}: oracle LL step {step:.2e}")
+        if glog["lldec"][kdiv] != logs["lldec"][kdiv]:
+            assert step <= 10 * ll_tol, "an LL-decrease decision differs without a near-tie"
+        return
     np.testing.assert_allclose(glog["lrate"], logs["lrate"], rtol=1e-12)
-    assert rel.max() <= ll_tol
     for h in range(M):
         Wg = gst["W"][h * n * n:(h + 1) * n * n].reshape(n, n).T
         Wo = fin["W"][h * n * n:(h + 1) * n * n].reshape(n, n).T
@@ -165,10 +208,10 @@ def test_graph_replay_matches_eager(built):
             eng.load_raw(x)
             eng.preprocess(1)
             eng.set_state(st0)
-            for _ in range(3):       # replays separated by host synchronisation
+            for k in range(3):       # replays separated by host reads of the logs
                 eng.iterate(2, graph=graph)
                 eng.sync()
-                eng.estep_stats() if not graph else None
+                eng.log(2 * (k + 1))
             res.append((eng.log(6), eng.get_state()))
     (la, sa), (lb, sb) = res
     np.testing.assert_array_equal(la["ll"], lb["ll"])
