@@ -70,12 +70,14 @@ struct amica_engine {
     float* xraw = nullptr;
     float* xw = nullptr;           /* float32 whitened data (both paths) */
     double* xw64 = nullptr;        /* FP64 whitened data (FP32 path) */
-    uint16_t* xbf = nullptr;       /* bf16 whitened data (BF16 path) */
+    uint16_t* xbf = nullptr;       /* bf16 whitened data, hi then lo part (BF16 path) */
+    uint16_t* x8 = nullptr;        /* bf16 hi part packed 8 samples per column (BF16 path) */
     float* b = nullptr;            /* FP32 unmixed data (BF16 path) */
     double* b64 = nullptr;         /* FP64 unmixed data (FP32 path) */
     void* g = nullptr;
     /* partials */
     int nblocks = 0;
+    int estep_wg_per_cu = 0;
     long chunk = 0;
     size_t pstride = 0;
     double* part = nullptr;
@@ -156,14 +158,14 @@ struct amica_engine {
         if (with_timing) HIPCHK(hipEventRecord(ev[0], st));
         LCHK(launch_prepare(ma, st));
         if (bf16)
-            LCHK(launch_unmix_bf16x3(xw, Whi, Wlo, wc, b, o.T, NP, ldb, st));
+            LCHK(launch_unmix_bf16x3(xw, xbf, Whi, Wlo, wc, b, o.T, NP, ldb, cus, st));
         else
             LCHK(launch_unmix_f64(xw64, Wg64, wc64, b64, o.T, NP, ldb, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
         LCHK(launch_estep(eargs(ltall_on ? ltall : nullptr), o.m, nblocks, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[2], st));
         if (bf16)
-            LCHK(launch_gx_bf16((const uint16_t*)g, xbf, gxpart, o.T, NP, ldb, nchunks, st));
+            LCHK(launch_gx_bf16((const uint16_t*)g, x8, gxpart, o.T, NP, ldb, nchunks, st));
         else
             LCHK(launch_gx_f32((const float*)g, xw, gxpart, o.T, NP, ldb, nchunks, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[3], st));
@@ -251,35 +253,57 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
 #define ALLOC(ptr, bytes) do { int _r = alloc((void**)&(ptr), (bytes)); if (_r) { amica_engine_destroy(e); return _r; } } while (0)
     ALLOC(e->xw, (size_t)T * NP * sizeof(float));
     if (e->bf16) {
-        ALLOC(e->xbf, (size_t)T * NP * sizeof(uint16_t));
+        ALLOC(e->xbf, 2 * (size_t)T * NP * sizeof(uint16_t));   /* hi, then lo */
         ALLOC(e->b, (size_t)T * e->ldb * sizeof(float));
     } else {
         ALLOC(e->xw64, (size_t)T * NP * sizeof(double));
         ALLOC(e->b64, (size_t)T * e->ldb * sizeof(double));
     }
-    ALLOC(e->g, (size_t)T * e->ldb * (e->bf16 ? sizeof(uint16_t) : sizeof(float)));
-    /* E-step grid: every CU holds 8 waves of this kernel (2 per SIMD). */
+    /* BF16: g and x packed [T32/8][cols][8] for the statistic GEMM; the
+     * padding samples up to T32 stay zero */
+    const long T32 = (T + 31) / 32 * 32;
+    if (e->bf16) {
+        ALLOC(e->x8, (size_t)T32 * NP * sizeof(uint16_t));
+        ALLOC(e->g, (size_t)T32 * e->ldb * sizeof(uint16_t));
+    } else {
+        ALLOC(e->g, (size_t)T * e->ldb * sizeof(float));
+    }
+    /* E-step grid: exactly as many workgroups as the CUs hold at once, each
+     * streaming one contiguous range of samples. */
     const int waves = M * CW;
-    const int wg_per_cu = std::max(1, 8 / waves);
-    long target = (long)e->cus * wg_per_cu;
-    long maxb = (T + 3) / 4;
-    e->nblocks = (int)std::max(1L, std::min(target, maxb));
-    e->chunk = (T + e->nblocks - 1) / e->nblocks;
-    e->chunk = ((e->chunk + 3) / 4) * 4;
-    e->nblocks = (int)((T + e->chunk - 1) / e->chunk);
-    const int NS = kStatsPerJ * m + kStatsPerI;
-    e->pstride = (size_t)M * NS * NP + M + 1;
-    e->pstride = (e->pstride + 31) & ~(size_t)31;
-    ALLOC(e->part, (size_t)e->nblocks * e->pstride * sizeof(double));
     {
-        const int rc = estep_configure(m, (size_t)NS * 64 * waves * sizeof(double));
+        const int rc = estep_configure(m, estep_lds_bytes(m, 64 * waves));
         if (rc) {
             amica_engine_destroy(e);
             return fail(AMICA_E_DEVICE, "E-step LDS configuration failed");
         }
     }
-    const int tiles = (e->ldb / 64) * (NP / 64);
-    e->nchunks = (int)std::max(1L, std::min((long)(2 * e->cus + tiles - 1) / tiles, (T + 31) / 32));
+    if (mstep_configure(n)) {
+        amica_engine_destroy(e);
+        return fail(AMICA_E_DEVICE, "M-step LDS configuration failed");
+    }
+    const int wg_per_cu = estep_blocks_per_cu(m, !e->bf16, 64 * waves);
+    if (wg_per_cu < 1) {
+        amica_engine_destroy(e);
+        return fail(AMICA_E_DEVICE, "E-step occupancy query failed");
+    }
+    e->estep_wg_per_cu = wg_per_cu;
+    long target = (long)e->cus * wg_per_cu;
+    long maxb = (T + 31) / 32;
+    e->nblocks = (int)std::max(1L, std::min(target, maxb));
+    e->chunk = (T + e->nblocks - 1) / e->nblocks;
+    e->chunk = ((e->chunk + 31) / 32) * 32;     /* whole packed 8-sample groups */
+    e->nblocks = (int)((T + e->chunk - 1) / e->chunk);
+    const int NS = kStatsPerJ * m + kStatsPerI;
+    e->pstride = (size_t)M * NS * NP + M + 1;
+    e->pstride = (e->pstride + 31) & ~(size_t)31;
+    ALLOC(e->part, (size_t)e->nblocks * e->pstride * sizeof(double));
+    if (e->bf16) {
+        e->nchunks = gx_bf16_chunks(T, NP, e->cus);
+    } else {
+        const int tiles = (e->ldb / 64) * (NP / 64);
+        e->nchunks = (int)std::max(1L, std::min((long)(2 * e->cus + tiles - 1) / tiles, (T + 31) / 32));
+    }
     ALLOC(e->gxpart, (size_t)e->nchunks * e->ldb * NP * sizeof(double));
     ALLOC(e->raw, e->pstride * sizeof(double));
     ALLOC(e->gxtot, (size_t)e->ldb * NP * sizeof(double));
@@ -338,7 +362,7 @@ int amica_engine_destroy(amica_engine* e) {
     if (e->st) (void)hipStreamSynchronize(e->st);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
-    void* ptrs[] = {e->xraw, e->xw, e->xw64, e->xbf, e->b, e->b64, e->g, e->part, e->gxpart,
+    void* ptrs[] = {e->xraw, e->xw, e->xw64, e->xbf, e->x8, e->b, e->b64, e->g, e->part, e->gxpart,
                     e->raw, e->gxtot, e->A, e->c, e->alpha, e->mu, e->beta, e->rho, e->gm, e->W,
                     e->ldet, e->lconst, e->Wg64, e->wc64, e->Whi, e->Wlo, e->wc, e->cst,
                     e->cst64, e->stats, e->sigma2, e->cnew, e->phis,
@@ -407,6 +431,7 @@ int amica_engine_whiten(amica_engine* e, int do_sphere) {
         if (launch_sphere_matrix(V, d, C, n, e->xraw, e->mean, T, do_sphere, e->S, e->ldetS_d,
                                  part, nchunks, e->st)) { rc = -1; break; }
         if (launch_whiten(e->xraw, e->mean, e->S, T, n, e->NP, e->xw, e->xw64, e->xbf, e->st)) { rc = -1; break; }
+        if (e->x8 && launch_pack8(e->xbf, T, e->NP, e->x8, e->st)) { rc = -1; break; }
     } while (0);
     const hipError_t se = hipStreamSynchronize(e->st);
     (void)hipFree(part); (void)hipFree(C); (void)hipFree(V); (void)hipFree(d);
@@ -441,6 +466,7 @@ int amica_engine_load_white(amica_engine* e, const double* xw, double ldetS) {
     HIPCHK(hipMalloc((void**)&tmp, cnt * sizeof(double)));
     HIPCHK(hipMemcpy(tmp, xw, cnt * sizeof(double), hipMemcpyHostToDevice));
     int rc = launch_white_from_f64(tmp, e->o.T, e->o.n, e->NP, e->xw, e->xw64, e->xbf, e->st);
+    if (!rc && e->x8) rc = launch_pack8(e->xbf, e->o.T, e->NP, e->x8, e->st);
     hipError_t se = hipStreamSynchronize(e->st);
     (void)hipFree(tmp);
     if (rc || se != hipSuccess) return fail(AMICA_E_DEVICE, "load_white failed");
@@ -659,10 +685,10 @@ int amica_engine_debug_buffer(amica_engine* e, const char* name, void* out, size
     const Buf bufs[] = {
         {"x", e->xw, T * NP * sizeof(float)},
         {"x64", e->xw64, e->xw64 ? T * NP * sizeof(double) : 0},
-        {"xbf", e->xbf, e->xbf ? T * NP * sizeof(uint16_t) : 0},
+        {"xbf", e->xbf, e->xbf ? 2 * T * NP * sizeof(uint16_t) : 0},
         {"b", e->b, e->b ? T * ldb * sizeof(float) : 0},
         {"b64", e->b64, e->b64 ? T * ldb * sizeof(double) : 0},
-        {"g", e->g, T * ldb * (e->bf16 ? sizeof(uint16_t) : sizeof(float))},
+        {"g", e->g, e->bf16 ? (T + 31) / 32 * 32 * ldb * sizeof(uint16_t) : T * ldb * sizeof(float)},
         {"gx", e->gxtot, ldb * NP * sizeof(double)},
         {"raw", e->raw, e->pstride * sizeof(double)},
         {"Wg64", e->Wg64, e->Wg64 ? ldb * NP * sizeof(double) : 0},

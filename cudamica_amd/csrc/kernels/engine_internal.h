@@ -45,13 +45,13 @@ struct EstepArgs {
     const double* b64;     /* [T][ldb] (FP64 unmixing), or null */
     const double* cst64;   /* [M][kNumCst64][m][NP] when b64 is set */
     const float* x;        /* [T][NP] whitened data */
-    void* g;               /* [T][ldb] fp32 or bf16 */
+    void* g;               /* fp32 [T][ldb], or bf16 [T32/8][ldb][8] (8 samples packed) */
     const float* cst;      /* [M][kNumCst][m][NP] */
     const double* lconst;  /* [M] log gm + ldet + ldetS */
     double* part;          /* [nblocks][pstride] */
     float* ltall;          /* optional [M][T], L_ht - max_h const (natural log) */
     long T;
-    long chunk;            /* samples per workgroup, multiple of 4 */
+    long chunk;            /* samples per workgroup, multiple of 8 */
     size_t pstride;
     int n, NP, M, CW, ldb;
     int g_bf16;
@@ -60,20 +60,33 @@ struct EstepArgs {
 int launch_estep(const EstepArgs& a, int m, int nblocks, hipStream_t st);
 /* Allows the E-step variants for m components `lds_bytes` of dynamic LDS. */
 int estep_configure(int m, size_t lds_bytes);
+/* Dynamic LDS of one E-step workgroup of `threads` threads. */
+size_t estep_lds_bytes(int m, int threads);
+/* Resident E-step workgroups per CU (registers and LDS), or -1. */
+int estep_blocks_per_cu(int m, bool b64, int threads);
 
 /* GEMMs ------------------------------------------------------------------ */
 /* b[t][c] = sum_k x[t][k] W[c][k] - wc[c]    (c < ldb, k < NP) */
 int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double* b,
                      long T, int NP, int ldb, hipStream_t st);
-/* x float32 split on the fly into bf16 hi + lo; W given as bf16 hi and lo:
- * b = x_hi W_hi + x_hi W_lo + x_lo W_hi (three MFMAs, ~2^-16 relative) */
-int launch_unmix_bf16x3(const float* x, const uint16_t* Whi, const uint16_t* Wlo,
-                        const float* wc, float* b, long T, int NP, int ldb, hipStream_t st);
+/* b = x_hi W_hi + x_hi W_lo + x_lo W_hi (three MFMAs, ~2^-16 relative) with
+ * W given as bf16 hi and lo, x as the split bf16 copy xbf ([T][NP] hi, then
+ * [T][NP] lo) or, for NP > 256, float32 x split on the fly. */
+int launch_unmix_bf16x3(const float* x, const uint16_t* xbf, const uint16_t* Whi,
+                        const uint16_t* Wlo, const float* wc, float* b, long T, int NP, int ldb,
+                        int cus, hipStream_t st);
 /* part[chunk][c][k] = sum_{t in chunk} g[t][c] x[t][k]   (FP64 partials) */
 int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, int ldb,
                   int nchunks, hipStream_t st);
-int launch_gx_bf16(const uint16_t* g, const uint16_t* x, double* part, long T, int NP, int ldb,
+/* BF16 path: g and x packed 8 samples per column ([T32/8][cols][8], T32 =
+ * T rounded up to 32, zero padded); part is [nchunks][ldb][NP]. */
+int launch_gx_bf16(const uint16_t* g8, const uint16_t* x8, double* part, long T, int NP, int ldb,
                    int nchunks, hipStream_t st);
+/* Number of sample chunks (partials) launch_gx_bf16 should be given. */
+int gx_bf16_chunks(long T, int NP, int cus);
+/* x8[(t/8) NP + i][t%8] = xhi[t][i] (the packed B operand of the BF16
+ * statistic GEMM); padding stays as allocated (zero). */
+int launch_pack8(const uint16_t* xhi, long T, int NP, uint16_t* x8, hipStream_t st);
 
 /* Reductions and the M-step ------------------------------------------------ */
 struct DevState {
@@ -125,8 +138,10 @@ struct MstepArgs {
 int launch_reduce(const double* part, int nparts, size_t pstride, const double* gxpart,
                   int nchunks, double* raw, double* gxtot, const MstepArgs& ma, hipStream_t st);
 int launch_mstep(const MstepArgs& ma, hipStream_t st);
-/* LU inverse of A -> W, ldet, GEMM-precision W, W c, E-step constants. */
+/* W = A^-1, ldet, GEMM-precision W, W c, E-step constants. */
 int launch_prepare(const MstepArgs& ma, hipStream_t st);
+/* Allows the LDS inverse its dynamic shared memory (once per engine). */
+int mstep_configure(int n);
 
 /* Preprocessing -------------------------------------------------------------- */
 int launch_colmean(const float* x, long T, int n, double* part, int nchunks, double* mean,

@@ -13,7 +13,9 @@
  *                                   computeu / computefp / computeg  kernels.cu:473-574
  *   usum, kappa, lambda, mu, beta, rho numerators         getusum ... getdrhonumer
  *   vsum, sigma2 = sum v b^2, cnew = sum v x               amica.cu:1280-1323, 1584
- * and writes g for the statistic GEMM.
+ * and writes g for the statistic GEMM (float32 [T][ldb], or bf16 packed 8
+ * samples per column, [T/8][ldb][8], the layout the BF16 statistic GEMM
+ * reads its MFMA fragments in).
  *
  * Layout: b and g are [T][ldb] with column h*NP + i; x is [T][NP]; NP is n
  * rounded up to 64.  Lane = channel, wave = (model h, 64-channel block cb);
@@ -23,7 +25,8 @@
  *   la  = log2|y|
  *   ym2 = 2^((rho-2) la) = |y|^(rho-2),  sy = ym2 y = sign(y)|y|^(rho-1),
  *   yr  = sy y = |y|^rho
- *   e_j = 2^(Q2_j - max Q2)  with Q2 = Q log2(e)
+ *   e_j = 2^Q2_j  with Q2 = Q log2(e) (shifted by max_j Q2 only when
+ *         every component underflows)
  * and every statistic becomes a product of these with u, with the constant
  * per-(h,j,i) factors (rho, sqrt(beta), ln 2) applied once in the reduction:
  *   usum   = sum u                       mu_numer  = rho  sum u sy
@@ -31,9 +34,16 @@
  *   beta_d = rho sum u yr                            sqrt(beta) rho^2 sum u sy^2 (rho > 2)
  *   lambda = rho^2 sum u yr^2 - 2 rho sum u yr + usum
  *   rho_n  = rho ln2 sum u yr la
- * Statistics are summed in FP32 over 32 samples, then in FP64 per lane (in
- * LDS), and written as one FP64 partial per workgroup (no atomics: the
- * reduction kernel sums partials in a fixed order).
+ * Statistics are summed in FP32 registers over 32 samples, those partials in
+ * FP32 per lane in LDS over 1024 samples, and those in FP64 in the
+ * workgroup's own slot of the partial buffer, which the reduction kernel
+ * sums over workgroups in a fixed order (no atomics, run-to-run identical).
+ *
+ * Latency: the b (and x) loads of tile k+1 are issued before tile k's math,
+ * the per-sample channel sums use DPP row shifts and row broadcasts instead
+ * of LDS permutes, one barrier per 4-sample tile exchanges the per-model
+ * sums, and the per-lane LDS footprint (FP32 level-2 totals) leaves room for
+ * two 512-thread workgroups per CU.
  *
  * B64 = true (FP32 path): b is FP64 and y = sqrt(beta)(b - mu) is formed in
  * FP64 before the FP32 density math, so |y| near 0 -- where mu_denom's
@@ -47,12 +57,38 @@ namespace amica {
 
 namespace {
 
-constexpr int kS = 4;          /* samples per barrier tile */
-constexpr int kFlush = 8;      /* tiles summed in FP32 before the FP64 flush */
+constexpr int kS = 2;          /* samples per barrier tile */
+constexpr int kFlush = 32 / kS; /* tiles summed in FP32 registers (32 samples) */
+constexpr int kFlush2 = 32;    /* register flushes summed in FP32 LDS */
 
 __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <int CTRL, int ROWS, int BANKS, bool BOUND>
+__device__ __forceinline__ float dpp(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 CTRL, ROWS, BANKS, BOUND));
+}
+
+/* Sum over the 64 lanes; the total lands in lane 63 (other lanes hold
+ * partial sums).  Row shifts 1, 2 (groups of 4), 4, 8 (row of 16, in lane
+ * 15 of each row), then row broadcasts 15 and 31 across rows. */
+__device__ __forceinline__ float wave_sum_lane63(float v) {
+    v += dpp<0x111, 0xf, 0xf, true>(v);
+    v += dpp<0x112, 0xf, 0xf, true>(v);
+    v += dpp<0x114, 0xf, 0xe, true>(v);
+    v += dpp<0x118, 0xf, 0xc, true>(v);
+    v += dpp<0x142, 0xa, 0xf, false>(v);
+    v += dpp<0x143, 0xc, 0xf, false>(v);
+    return v;
+}
+
+/* two floats -> two round-to-nearest-even bf16 in one dword (a low) */
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)a) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)b) << 16);
+}
 
 template <int MJ, bool B64>
 __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
@@ -64,12 +100,15 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     const int cb = w - h * a.CW;
     const int i = cb * 64 + lane;
     const bool act = i < a.n;
-    const int nw = a.M * a.CW;
 
-    __shared__ float Lp[2][kMaxEstepWaves][kS];
+    /* per-wave channel sums of one tile, double buffered: [buf][wave][sample] */
+    __shared__ __align__(16) float Lp[2][kMaxEstepWaves][kS];
     __shared__ float dc2[AMICA_MAX_M];
-    /* per-lane FP64 totals, [NS][blockDim.x] */
-    extern __shared__ double tot[];
+    /* per-lane FP32 level-2 totals, [blockDim.x][NSP] (odd stride: the
+     * lanes of a wave hit distinct banks) */
+    constexpr int NSP = NS | 1;
+    extern __shared__ float tot2_all[];
+    float* const tot2 = tot2_all + threadIdx.x * NSP;
 
     /* L'_h = (log gm + ldet + ldetS) - c0 with c0 = max_h of that constant,
      * so that only differences enter FP32 (in log2 units). */
@@ -101,95 +140,144 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
         }
     }
 #pragma unroll
-    for (int s = 0; s < NS; ++s) tot[s * blockDim.x + threadIdx.x] = 0.0;
-    __syncthreads();
+    for (int s = 0; s < NS; ++s) tot2[s] = 0.f;
 
     float acc[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) acc[s] = 0.f;
     double vsum_acc = 0.0, ll_acc = 0.0;
+    double* const base = a.part + (size_t)blockIdx.x * a.pstride;
+    /* this lane's FP64 totals: [M][NP][NS], contiguous per lane */
+    double* const l3 = base + ((size_t)h * a.NP + i) * NS;
+    bool l3_started = false;
 
     const long t0 = (long)blockIdx.x * a.chunk;
     const long t1 = min(a.T, t0 + a.chunk);
     const size_t bcol = (size_t)h * a.NP + i;
-    int buf = 0, tiles = 0;
+    int buf = 0, tiles = 0, flushes = 0;
+    /* bf16 g, 8 samples packed per column: [t/8][ldb][8].  The words of the
+     * current 8-sample group (2 samples each) shift through gw0..gw3. */
+    uint32_t gw0 = 0, gw1 = 0, gw2 = 0, gw3 = 0;
+    int gq = 0;
+    long gbase = t0;
+    uint4* const g8 = (uint4*)a.g;
+
+    /* the loads of the first tile */
+    float bn[kS];
+    double bn64[B64 ? kS : 1];
+#pragma unroll
+    for (int s = 0; s < kS; ++s) {
+        const long t = t0 + s;
+        const bool ok = act && t < t1;
+        if constexpr (B64) bn64[s] = ok ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
+        else bn[s] = ok ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
+    }
 
     for (long tb = t0; tb < t1; tb += kS) {
         float E[kS][MJ], SY[kS][MJ], YM2[kS][MJ], YR[kS][MJ], LA[kS][MJ];
-        float RS[kS], BV[kS];
-        /* ---- phase 1: densities, component posteriors, channel sums ---- */
+        float RS[kS], BV[kS], XV[kS];
+        double BV64[B64 ? kS : 1];
 #pragma unroll
         for (int s = 0; s < kS; ++s) {
+            if constexpr (B64) { BV64[s] = bn64[s]; BV[s] = (float)bn64[s]; }
+            else BV[s] = bn[s];
+        }
+        /* issue the next tile's b loads and this tile's x loads now; they
+         * complete under this tile's math */
+#pragma unroll
+        for (int s = 0; s < kS; ++s) {
+            const long tn = tb + kS + s;
+            const bool okn = act && tn < t1;
+            if constexpr (B64) bn64[s] = okn ? a.b64[(size_t)tn * a.ldb + bcol] : 0.0;
+            else bn[s] = okn ? a.b[(size_t)tn * a.ldb + bcol] : 0.f;
             const long t = tb + s;
-            const bool valid = t < t1;
-            float bv;
-            double bv64 = 0.0;
-            if constexpr (B64) {
-                bv64 = (act && valid) ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
-                bv = (float)bv64;
-            } else {
-                bv = (act && valid) ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
-            }
-            BV[s] = bv;
+            XV[s] = (act && t < t1) ? a.x[(size_t)t * a.NP + i] : 0.f;
+        }
+        /* ---- phase 1: densities, component posteriors, channel sums ---- */
+        float QS[kS];
+#pragma unroll
+        for (int s = 0; s < kS; ++s) {
             float q2[MJ];
-            float mx = 0.f;
 #pragma unroll
             for (int j = 0; j < MJ; ++j) {
                 float y;
-                if constexpr (B64) y = (float)fma(sb64[j], bv64, nsbmu64[j]);
-                else y = fmaf(sb[j], bv, nsbmu[j]);
+                if constexpr (B64) y = (float)fma(sb64[j], BV64[s], nsbmu64[j]);
+                else y = fmaf(sb[j], BV[s], nsbmu[j]);
                 const float la = fast_log2(fmaxf(fabsf(y), 1.17549435e-38f));
                 const float ym2 = fast_exp2(rm2[j] * la);
                 const float sy = ym2 * y;
                 const float yr = sy * y;
                 q2[j] = fmaf(yr, -AMICA_L2E, cq2[j]);
-                mx = (j == 0) ? q2[j] : fmaxf(mx, q2[j]);
                 SY[s][j] = sy; YM2[s][j] = ym2; YR[s][j] = yr; LA[s][j] = la;
             }
-            float se = 0.f;
+            /* q2 <= log2(alpha sqrt(beta) / (2 Gamma(1 + 1/rho))), a few
+             * units, so 2^q2 needs no max shift unless every component
+             * underflows (a sample far from all of them): then, for the
+             * whole wave, shift by the per-lane max */
+            float se = 0.f, off = 0.f;
 #pragma unroll
             for (int j = 0; j < MJ; ++j) {
-                E[s][j] = fast_exp2(q2[j] - mx);
+                E[s][j] = fast_exp2(q2[j]);
                 se += E[s][j];
             }
+            if (__builtin_expect(__any(se < 0x1p-100f), 0)) {
+                off = q2[0];
+#pragma unroll
+                for (int j = 1; j < MJ; ++j) off = fmaxf(off, q2[j]);
+                se = 0.f;
+#pragma unroll
+                for (int j = 0; j < MJ; ++j) {
+                    E[s][j] = fast_exp2(q2[j] - off);
+                    se += E[s][j];
+                }
+            }
             RS[s] = fast_rcp(se);
-            const float qtot = mx + fast_log2(se);
-            float qs = act ? qtot : 0.f;
-            qs = wave_sum_f32(qs);
-            if (lane == 0) Lp[buf][w][s] = qs;
+            QS[s] = act ? off + fast_log2(se) : 0.f;
+        }
+#pragma unroll
+        for (int s = 0; s < kS; ++s) QS[s] = wave_sum_lane63(QS[s]);
+        if (lane == 63) {
+#pragma unroll
+            for (int s = 0; s < kS; ++s) Lp[buf][w][s] = QS[s];
         }
         __syncthreads();
-        /* ---- model posteriors: lane s < kS handles sample tb+s ---- */
+        /* ---- model posteriors: lane hh*kS + s holds model hh, sample s ---- */
         float vl = 0.f;
-        if (lane < kS) {
-            const long t = tb + lane;
+        {
+            const int hh = lane / kS;
+            const int s = lane - hh * kS;
+            const long t = tb + s;
             const bool valid = t < t1;
-            float Lmine = 0.f, Lmax = -3.0e38f;
-            for (int hh = 0; hh < a.M; ++hh) {
-                float l = dc2[hh];
-                for (int c = 0; c < a.CW; ++c) l += Lp[buf][hh * a.CW + c][lane];
-                Lmax = fmaxf(Lmax, l);
-                if (hh == h) Lmine = l;
+            float l = -3.0e38f;
+            if (hh < a.M) {
+                l = dc2[hh];
+                for (int c = 0; c < a.CW; ++c) l += Lp[buf][hh * a.CW + c][s];
             }
-            float sum = 0.f;
-            for (int hh = 0; hh < a.M; ++hh) {
-                float l = dc2[hh];
-                for (int c = 0; c < a.CW; ++c) l += Lp[buf][hh * a.CW + c][lane];
-                sum += fast_exp2(l - Lmax);
-            }
-            const float P2 = Lmax + fast_log2(sum);
-            vl = valid ? fast_exp2(Lmine - P2) : 0.f;
-            if (valid) {
-                if (cb == 0) vsum_acc += (double)vl;
-                if (h == 0 && cb == 0) ll_acc += (double)P2 * AMICA_LN2;
-                if (a.ltall && cb == 0) a.ltall[(size_t)h * a.T + t] = Lmine * (float)AMICA_LN2;
+            /* logsumexp over the models: the lanes of one sample are kS = 2
+             * apart inside the first row of 16 lanes (M <= 8), so row
+             * rotations by 2, 4 and 8 reach all of them */
+            static_assert(kS == 2, "the model reduction assumes 2 samples per tile");
+            float lmax = fmaxf(l, dpp<0x122, 0xf, 0xf, false>(l));      /* row_ror:2 */
+            lmax = fmaxf(lmax, dpp<0x124, 0xf, 0xf, false>(lmax));      /* row_ror:4 */
+            lmax = fmaxf(lmax, dpp<0x128, 0xf, 0xf, false>(lmax));      /* row_ror:8 */
+            const float e = (hh < a.M) ? fast_exp2(l - lmax) : 0.f;
+            float esum = e + dpp<0x122, 0xf, 0xf, false>(e);
+            esum += dpp<0x124, 0xf, 0xf, false>(esum);
+            esum += dpp<0x128, 0xf, 0xf, false>(esum);
+            const float P2 = lmax + fast_log2(esum);
+            vl = (valid && hh < a.M) ? fast_exp2(l - P2) : 0.f;
+            if (w == 0 && valid && hh < a.M) {
+                vsum_acc += (double)vl;
+                if (hh == 0) ll_acc += (double)P2 * AMICA_LN2;
+                if (a.ltall) a.ltall[(size_t)hh * a.T + t] = l * (float)AMICA_LN2;
             }
         }
         /* ---- phase 2: responsibilities and statistics ---- */
+        float GV[kS];
 #pragma unroll
         for (int s = 0; s < kS; ++s) {
             const long t = tb + s;
-            const float v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vl), s));
+            const float v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vl), h * kS + s));
             const float vrs = v * RS[s];
             float g = 0.f;
 #pragma unroll
@@ -207,43 +295,63 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 aj[5] = fmaf(uyr, YR[s][j], aj[5]);
                 aj[6] = fmaf(uyr, LA[s][j], aj[6]);
             }
-            if (t < t1) {
-                const float vb = v * BV[s];
-                acc[NSJ * MJ + 0] = fmaf(vb, BV[s], acc[NSJ * MJ + 0]);
-                const float xv = act ? a.x[(size_t)t * a.NP + i] : 0.f;
-                acc[NSJ * MJ + 1] = fmaf(v, xv, acc[NSJ * MJ + 1]);
-                if (!act) g = 0.f;
-                acc[NSJ * MJ + 2] += g;
-                if (a.g_bf16) {
-                    ((uint16_t*)a.g)[(size_t)t * a.ldb + bcol] = f32_to_bf16_bits(g);
-                } else {
-                    ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
-                }
+            if (!act) g = 0.f;
+            const float vb = v * BV[s];
+            acc[NSJ * MJ + 0] = fmaf(vb, BV[s], acc[NSJ * MJ + 0]);
+            acc[NSJ * MJ + 1] = fmaf(v, XV[s], acc[NSJ * MJ + 1]);
+            acc[NSJ * MJ + 2] += g;
+            GV[s] = (t < t1) ? g : 0.f;
+            if (!a.g_bf16 && t < t1) ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
+        }
+        if (a.g_bf16) {
+            static_assert(kS == 2, "one packed dword per tile");
+            gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = pack_bf16x2(GV[0], GV[1]);
+            if (++gq == 4) {
+                g8[(size_t)(gbase >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
+                gq = 0;
+                gbase = tb + kS;
             }
         }
         buf ^= 1;
         if (++tiles == kFlush) {
+            tiles = 0;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                tot[s * blockDim.x + threadIdx.x] += (double)acc[s];
+                tot2[s] += acc[s];
                 acc[s] = 0.f;
             }
-            tiles = 0;
+            if (++flushes == kFlush2) {
+                flushes = 0;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const double v = (double)tot2[s];
+                    l3[s] = l3_started ? l3[s] + v : v;
+                    tot2[s] = 0.f;
+                }
+                l3_started = true;
+            }
         }
     }
-    double* base = a.part + (size_t)blockIdx.x * a.pstride;
+    if (a.g_bf16 && gq != 0) {   /* a partial group at the end of the recording */
+        while (gq < 4) { gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = 0u; ++gq; }
+        g8[(size_t)(gbase >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
+    }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        const double v = tot[s * blockDim.x + threadIdx.x] + (double)acc[s];
-        base[((size_t)h * NS + s) * a.NP + i] = act ? v : 0.0;
+        const double v = (double)tot2[s] + (double)acc[s];
+        const double r = l3_started ? l3[s] + v : v;
+        l3[s] = act ? r : 0.0;
     }
-    /* per-model vsum and the log-likelihood sum from lanes 0..kS-1 */
-    double vs = wave_sum_f64(lane < kS ? vsum_acc : 0.0);
-    double ls = wave_sum_f64(lane < kS ? ll_acc : 0.0);
-    const size_t tail = (size_t)a.M * NS * a.NP;
-    if (lane == 0 && cb == 0) base[tail + h] = vs;
-    if (lane == 0 && cb == 0 && h == 0) base[tail + a.M] = ls;
-    (void)nw;
+    /* per-model vsum (lanes hh*kS + s of wave 0) and the LL sum (lanes s) */
+    if (w == 0) {
+        const size_t tail = (size_t)a.M * NS * a.NP;
+        for (int hh = 0; hh < a.M; ++hh) {
+            const double vs = wave_sum_f64((lane / kS == hh) ? vsum_acc : 0.0);
+            if (lane == 0) base[tail + hh] = vs;
+        }
+        const double ls = wave_sum_f64(ll_acc);
+        if (lane == 0) base[tail + a.M] = ls;
+    }
 }
 
 }  // namespace
@@ -273,9 +381,40 @@ int estep_configure(int MJ, size_t lds_bytes) {
     return r == hipSuccess ? 0 : -1;
 }
 
+size_t estep_lds_bytes(int MJ, int threads) {
+    return (size_t)((kStatsPerJ * MJ + kStatsPerI) | 1) * threads * sizeof(float);
+}
+
+int estep_blocks_per_cu(int MJ, bool b64, int threads) {
+    int nb = 0;
+    const size_t shm = estep_lds_bytes(MJ, threads);
+    hipError_t r = hipErrorInvalidValue;
+#define AMICA_ESTEP_OCC(J)                                                                        \
+    case J:                                                                                       \
+        r = b64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_kernel<J, true>,        \
+                                                               threads, shm)                      \
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_kernel<J, false>,       \
+                                                               threads, shm);                     \
+        break;
+    switch (MJ) {
+        AMICA_ESTEP_OCC(1)
+        AMICA_ESTEP_OCC(2)
+        AMICA_ESTEP_OCC(3)
+        AMICA_ESTEP_OCC(4)
+        AMICA_ESTEP_OCC(5)
+        AMICA_ESTEP_OCC(6)
+        AMICA_ESTEP_OCC(7)
+        AMICA_ESTEP_OCC(8)
+        default: return -1;
+    }
+#undef AMICA_ESTEP_OCC
+    return r == hipSuccess ? nb : -1;
+}
+
 int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
+    if (a.M * a.CW > kMaxEstepWaves || a.M > 8 || (a.chunk % 8) != 0) return -1;
     dim3 grid(nblocks), block(64 * a.M * a.CW);
-    const size_t shm = (size_t)(kStatsPerJ * MJ + kStatsPerI) * block.x * sizeof(double);
+    const size_t shm = estep_lds_bytes(MJ, (int)block.x);
     const bool b64 = a.b64 != nullptr;
 #define AMICA_ESTEP_CASE(J)                                                                   \
     case J:                                                                                   \

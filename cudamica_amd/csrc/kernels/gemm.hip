@@ -23,6 +23,8 @@
  * Both are checked end to end against the CPU oracle by the statistics
  * parity tests (tests/test_gpu_engine.py).
  */
+#include <algorithm>
+
 #include "common.h"
 #include "engine_internal.h"
 
@@ -161,6 +163,85 @@ __global__ __launch_bounds__(256) void unmix_bf16x3_kernel(const float* __restri
     }
 }
 
+/* Register-resident split-BF16 unmixing.  x is pre-split (xhi = bf16(x),
+ * xlo = bf16(x - xhi), written once per recording by the whitening kernel),
+ * so the product needs no conversions.  Wave w of a workgroup owns columns
+ * [16 CT w, 16 CT (w+1)) and keeps their W_hi / W_lo B-fragments for all
+ * NP = 32 KS <= 128 inputs in registers (CT KS 8 VGPRs) for the whole launch; the
+ * waves of a workgroup walk the same 16-sample tiles (tile = blockIdx.x,
+ * += gridDim.x), so each x tile comes from HBM once and from L1/L2 for the
+ * other waves, and the A fragments of the next tile are loaded while the
+ * current tile's 3 CT KS MFMAs run.  HBM traffic per launch: x hi+lo
+ * (4 NP bytes per sample) + b (4 ldb bytes per sample). */
+template <int CT, int KS>
+__global__ __launch_bounds__(512) void unmix_bf16x3_reg_kernel(const uint16_t* __restrict__ xhi,
+                                                                const uint16_t* __restrict__ xlo,
+                                                                const uint16_t* __restrict__ Whi,
+                                                                const uint16_t* __restrict__ Wlo,
+                                                                const float* __restrict__ wc,
+                                                                float* __restrict__ b, long T,
+                                                                int ldb, long ntiles) {
+    constexpr int NP = 32 * KS;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c0 = w * 16 * CT;
+    const int kofs = (lane >> 4) * 8;
+    i16x8 bhi[CT][KS], blo[CT][KS];
+    float bias[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const int col = c0 + c * 16 + (lane & 15);
+        bias[c] = wc[col];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            const size_t wofs = (size_t)col * NP + k * 32 + kofs;
+            bhi[c][k] = *(const i16x8*)(Whi + wofs);
+            blo[c][k] = *(const i16x8*)(Wlo + wofs);
+        }
+    }
+    i16x8 ahi[KS], alo[KS];
+    auto load_a = [&](long tile) {
+        const long t = tile * 16 + (lane & 15);
+        const bool ok = tile < ntiles && t < T;
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            const size_t xofs = (size_t)t * NP + k * 32 + kofs;
+            ahi[k] = ok ? *(const i16x8*)(xhi + xofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
+            alo[k] = ok ? *(const i16x8*)(xlo + xofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
+        }
+    };
+    long tile = blockIdx.x;
+    load_a(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        i16x8 chi[KS], clo[KS];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) { chi[k] = ahi[k]; clo[k] = alo[k]; }
+        load_a(tile + gridDim.x);
+        f32x4 acc[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < KS; ++k)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, clo[k]),
+                                                                 __builtin_bit_cast(bf16x8, bhi[c][k]), acc[c], 0, 0, 0);
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, chi[k]),
+                                                                 __builtin_bit_cast(bf16x8, blo[c][k]), acc[c], 0, 0, 0);
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, chi[k]),
+                                                                 __builtin_bit_cast(bf16x8, bhi[c][k]), acc[c], 0, 0, 0);
+            }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const int col = c0 + c * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long t = tile * 16 + (lane >> 4) * 4 + r;
+                if (t < T) b[(size_t)t * ldb + col] = acc[c][r] - bias[c];
+            }
+        }
+    }
+}
+
 /* ---------------- statistic GEMM G = g^T x, FP32 --------------------- */
 /* The reduction index is the sample, the row index of both operands, so
  * the 16x16x4 fragments are coalesced rows read straight from global
@@ -214,72 +295,86 @@ __global__ __launch_bounds__(256) void gx_f32_kernel(const float* __restrict__ g
 }
 
 /* ---------------- statistic GEMM G = g^T x, BF16 --------------------- */
-/* The reduction index (sample) is the row index of g and x, so fragments of
- * 8 consecutive samples are transposed through LDS: 32 samples x 64 columns
- * of g and of x per stage, stored [column][sample] so every lane reads its
- * 8 bf16 with one 16-byte LDS load. */
-__global__ __launch_bounds__(256) void gx_bf16_kernel(const uint16_t* __restrict__ g,
-                                                      const uint16_t* __restrict__ x,
-                                                      double* __restrict__ part, long T, int NP,
-                                                      int ldb, long chunk) {
-    __shared__ __attribute__((aligned(16))) uint16_t gs[64][40];
-    __shared__ __attribute__((aligned(16))) uint16_t xs[64][40];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int r0 = blockIdx.x * 64;
-    const int k0 = blockIdx.y * 64;
-    const long ts = (long)blockIdx.z * chunk;
-    const long te = min(T, ts + chunk);
-    f32x4 acc[4];
-    f64x4 tot[4];
+/* The reduction index (the sample) is the slow axis of g and x in sample-
+ * major memory, and an MFMA fragment wants 8 consecutive samples of one
+ * column per lane.  So both operands are stored packed 8 samples per column
+ * ([t/8][col][8] bf16: the E-step writes g that way, x is packed once per
+ * recording) and every A and B fragment is one 16-byte load straight from
+ * HBM, no LDS.  Workgroup = ldb/64 waves over one chunk of samples; wave w
+ * owns g rows [64w, 64w+64) x 16 CT x columns (grid.y splits NP into
+ * 16 CT-column slabs), 4 CT 16x16 FP32 accumulators, and the next 32-sample
+ * block's fragments are loaded while the current block's MFMAs run.  Every
+ * g and x element is read from HBM once per slab; FP32 accumulation inside
+ * the MFMA (32 products per rounding) over the chunk, FP64 partial per
+ * chunk. */
+template <int CT>
+__global__ __launch_bounds__(512) void gx_bf16_packed_kernel(const uint16_t* __restrict__ g8,
+                                                             const uint16_t* __restrict__ x8,
+                                                             double* __restrict__ part, long nblk,
+                                                             int NP, int ldb, long cblk) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r0 = w * 64;
+    const int k0 = blockIdx.y * 16 * CT;
+    const long b0 = (long)blockIdx.x * cblk;
+    const long b1 = min(nblk, b0 + cblk);
+    f32x4 acc[4][CT];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        acc[a] = (f32x4){0.f, 0.f, 0.f, 0.f};
-        tot[a] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    i16x8 A[4], B[CT];
+    auto load = [&](long blk) {
+        const bool ok = blk < b1;
+        const size_t grp = (size_t)(ok ? blk : b0) * 4 + (lane >> 4);
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+            A[rt] = *(const i16x8*)(g8 + (grp * ldb + r0 + rt * 16 + (lane & 15)) * 8);
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+            B[ct] = *(const i16x8*)(x8 + (grp * NP + k0 + ct * 16 + (lane & 15)) * 8);
+    };
+    if (b0 < b1) load(b0);
+    for (long blk = b0; blk < b1; ++blk) {
+        i16x8 Ac[4], Bc[CT];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) Ac[rt] = A[rt];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) Bc[ct] = B[ct];
+        load(blk + 1);
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+                acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, Ac[rt]),
+                                                                      __builtin_bit_cast(bf16x8, Bc[ct]),
+                                                                      acc[rt][ct], 0, 0, 0);
     }
-    int steps = 0;
-    /* load mapping: thread -> (sample row lr = tid/8, 8 columns at lc = (tid%8)*8) */
-    const int lr = tid >> 3, lc = (tid & 7) * 8;
-    for (long t = ts; t < te; t += 32) {
-        const long tr = t + lr;
-        i16x8 gv = (i16x8){0, 0, 0, 0, 0, 0, 0, 0}, xv = gv;
-        if (tr < te) {
-            gv = *(const i16x8*)(g + (size_t)tr * ldb + r0 + lc);
-            xv = *(const i16x8*)(x + (size_t)tr * NP + k0 + lc);
-        }
+    double* out = part + (size_t)blockIdx.x * ldb * NP;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            gs[lc + e][lr] = (uint16_t)gv[e];
-            xs[lc + e][lr] = (uint16_t)xv[e];
-        }
-        __syncthreads();
-        /* wave w: x columns [16w, 16w+16), all four 16-row tiles of g */
-        const i16x8 bv = *(const i16x8*)&xs[w * 16 + (lane & 15)][(lane >> 4) * 8];
+    for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const i16x8 av = *(const i16x8*)&gs[a * 16 + (lane & 15)][(lane >> 4) * 8];
-            acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
-                                                             __builtin_bit_cast(bf16x8, bv),
-                                                             acc[a], 0, 0, 0);
-        }
-        __syncthreads();
-        if (++steps == 4) {
+        for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) tot[a][r] += (double)acc[a][r];
-                acc[a] = (f32x4){0.f, 0.f, 0.f, 0.f};
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + rt * 16 + (lane >> 4) * 4 + r;
+                out[(size_t)row * NP + k0 + ct * 16 + (lane & 15)] = (double)acc[rt][ct][r];
             }
-            steps = 0;
-        }
+}
+
+__global__ void pack8_kernel(const uint16_t* __restrict__ xhi, long T, int NP,
+                             uint16_t* __restrict__ x8) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;   /* (group, column) */
+    const size_t ngrp = (size_t)(T + 7) / 8;
+    if (q >= ngrp * NP) return;
+    const size_t grp = q / NP;
+    const int i = (int)(q % NP);
+    i16x8 v = (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const long t = (long)grp * 8 + e;
+        if (t < T) v[e] = (short)xhi[(size_t)t * NP + i];
     }
-    double* out = part + (size_t)blockIdx.z * ldb * NP;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = r0 + a * 16 + (lane >> 4) * 4 + r;
-            out[(size_t)row * NP + k0 + w * 16 + (lane & 15)] = tot[a][r] + (double)acc[a][r];
-        }
+    *(i16x8*)(x8 + q * 8) = v;
 }
 
 }  // namespace
@@ -292,9 +387,29 @@ int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_unmix_bf16x3(const float* x, const uint16_t* Whi, const uint16_t* Wlo, const float* wc,
-                        float* b, long T, int NP, int ldb, hipStream_t st) {
+int launch_unmix_bf16x3(const float* x, const uint16_t* xbf, const uint16_t* Whi,
+                        const uint16_t* Wlo, const float* wc, float* b, long T, int NP, int ldb,
+                        int cus, hipStream_t st) {
     if (NP % 32 || ldb % 64) return -1;
+    /* register-resident W: 64 columns per wave, NP <= 128, ldb <= 512 */
+    constexpr int CT = 4;
+    const int waves = ldb / (16 * CT);
+    if (xbf && NP <= 128 && waves <= 8) {
+        const long ntiles = (T + 15) / 16;
+        const long grid = std::min<long>(ntiles, (long)cus * std::max(1, 8 / waves));
+        const uint16_t* xlo = xbf + (size_t)T * NP;
+        const dim3 g((unsigned)grid), blk(64 * waves);
+#define AMICA_UNMIX_REG(C, K)                                                                   \
+        hipLaunchKernelGGL((unmix_bf16x3_reg_kernel<C, K>), g, blk, 0, st, xbf, xlo, Whi, Wlo,   \
+                           wc, b, T, ldb, ntiles)
+        switch (NP) {
+            case 64: AMICA_UNMIX_REG(4, 2); break;
+            case 128: AMICA_UNMIX_REG(4, 4); break;
+            default: return -1;
+        }
+#undef AMICA_UNMIX_REG
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
     dim3 grid((unsigned)((T + 63) / 64), ldb / 64);
     hipLaunchKernelGGL(unmix_bf16x3_kernel, grid, dim3(256), 0, st, x, Whi, Wlo, wc, b, T, NP, ldb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -314,12 +429,31 @@ int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, 
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_gx_bf16(const uint16_t* g, const uint16_t* x, double* part, long T, int NP, int ldb,
+static int gx_bf16_ct(int NP) { return NP % 128 == 0 ? 8 : 4; }
+
+int gx_bf16_chunks(long T, int NP, int cus) {
+    const long nblk = (T + 31) / 32;
+    const int slabs = NP / (16 * gx_bf16_ct(NP));
+    return (int)std::max(1L, std::min(nblk, (long)std::max(1, cus / slabs)));
+}
+
+int launch_gx_bf16(const uint16_t* g8, const uint16_t* x8, double* part, long T, int NP, int ldb,
                    int nchunks, hipStream_t st) {
-    if (NP % 64 || ldb % 64) return -1;
-    dim3 grid(ldb / 64, NP / 64, nchunks);
-    hipLaunchKernelGGL(gx_bf16_kernel, grid, dim3(256), 0, st, g, x, part, T, NP, ldb,
-                       gx_chunk(T, nchunks, 32));
+    if (NP % 64 || ldb % 64 || ldb > 512 || nchunks < 1) return -1;
+    const int CT = gx_bf16_ct(NP);
+    const long nblk = (T + 31) / 32;
+    const long cblk = (nblk + nchunks - 1) / nchunks;
+    dim3 grid(nchunks, NP / (16 * CT)), block(ldb);   /* ldb / 64 waves */
+    if (CT == 8)
+        hipLaunchKernelGGL(gx_bf16_packed_kernel<8>, grid, block, 0, st, g8, x8, part, nblk, NP, ldb, cblk);
+    else
+        hipLaunchKernelGGL(gx_bf16_packed_kernel<4>, grid, block, 0, st, g8, x8, part, nblk, NP, ldb, cblk);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_pack8(const uint16_t* xhi, long T, int NP, uint16_t* x8, hipStream_t st) {
+    const size_t cnt = (size_t)((T + 7) / 8) * NP;
+    hipLaunchKernelGGL(pack8_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, xhi, T, NP, x8);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

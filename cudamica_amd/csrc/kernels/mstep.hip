@@ -16,9 +16,10 @@
  *   reparameterisation         src/amica.cu:2367-2447, kernels.cu:1023-1065
  *   W = pinv(A), ldet          src/amica.cu:27-127, 835-924
  * The reference computes pinv with a host SVD; for the non-singular A of an
- * EM run pinv(A) = A^-1, computed here by LU with partial pivoting, and
- * log|det W| = -sum log|u_kk| (the reference multiplies the diagonal of the
- * LU of W, which can overflow).
+ * EM run pinv(A) = A^-1, computed here by Gauss-Jordan elimination in LDS
+ * (n <= 128) or LU in global memory (larger n), both with partial pivoting,
+ * and log|det W| = -sum log|pivot| (the reference multiplies the diagonal
+ * of the LU of W, which can overflow).
  */
 #include "common.h"
 #include "engine_internal.h"
@@ -92,17 +93,18 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
     const int NS = kStatsPerJ * m + kStatsPerI;
     const COff o(n, m);
     double* st = a.stats + (size_t)h * a.stride;
-    const double* rh = raw + (size_t)h * NS * NP;
+    /* raw statistics are [M][NP][NS]: stat s of (h, i) at (h NP + i) NS + s */
+    const double* rh = raw + (size_t)h * NP * NS;
     const long nm = (long)n * m;
     if (blockIdx.y == 0) {
         for (long q = threadIdx.x; q < nm; q += blockDim.x) {
             const int j = (int)(q / n), i = (int)(q % n);
             const double be = a.beta[(size_t)h * nm + q], r = a.rho[(size_t)h * nm + q];
             const double sb = sqrt(be);
-            const double* rj = rh + (size_t)j * kStatsPerJ * NP + i;
-            const double U = rj[kRawU * NP], Us = rj[kRawUs * NP], Us2 = rj[kRawUs2 * NP];
-            const double Um2 = rj[kRawUm2 * NP], Ur = rj[kRawUr * NP], Ur2 = rj[kRawUr2 * NP];
-            const double Url = rj[kRawUrl * NP];
+            const double* rj = rh + (size_t)i * NS + j * kStatsPerJ;
+            const double U = rj[kRawU], Us = rj[kRawUs], Us2 = rj[kRawUs2];
+            const double Um2 = rj[kRawUm2], Ur = rj[kRawUr], Ur2 = rj[kRawUr2];
+            const double Url = rj[kRawUrl];
             st[o.usum + q] = U;
             st[o.kn + q] = be * r * r * Us2;
             st[o.ln + q] = r * r * Ur2 - 2.0 * r * Ur + U;
@@ -112,10 +114,10 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
             st[o.bd + q] = r * Ur;
             st[o.rn + q] = r * AMICA_LN2 * Url;
         }
-        const double* ri = rh + (size_t)kStatsPerJ * m * NP;
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            st[o.s2 + i] = ri[kRawS2 * NP + i];
-            st[o.cnew + i] = ri[kRawCnew * NP + i];
+            const double* ri = rh + (size_t)i * NS + kStatsPerJ * m;
+            st[o.s2 + i] = ri[kRawS2];
+            st[o.cnew + i] = ri[kRawCnew];
         }
         if (threadIdx.x == 0) {
             const size_t tail = (size_t)a.M * NS * NP;
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
         }
     }
     /* phi_h[i][k] = sum_l G[h*NP+i][l] W~_h[k][l] - gsum_h[i] wc_h[k] */
-    const double* gsum = rh + (size_t)(kStatsPerJ * m + kRawGsum) * NP;
+    const double* gsum_base = rh + kStatsPerJ * m + kRawGsum;
     const int rows_per = (n + gridDim.y - 1) / gridDim.y;
     const int ib = blockIdx.y * rows_per, ie = min(n, ib + rows_per);
     const long cnt = (long)(ie - ib) * n;
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
             for (int l = 0; l < n; ++l) s += grow[l] * a.Wg64[wrow + l];
             wck = a.wc64[(size_t)h * NP + k];
         }
-        s -= gsum[i] * wck;
+        s -= gsum_base[(size_t)i * NS] * wck;
         st[o.phi + i + (long)k * n] = s;
     }
 }
@@ -288,8 +290,25 @@ __global__ void mstep_b_kernel(MstepArgs a) {
 }
 
 /* A update, reparameterisation and c <- cnew for model blockIdx.x. */
-__global__ __launch_bounds__(kBlk) void mstep_c_kernel(MstepArgs a) {
+/* AB_h = A_h B_h for every model, one thread per output element (the
+ * threads of a wave share the column of B and read consecutive rows of A). */
+__global__ __launch_bounds__(256) void ab_kernel(MstepArgs a) {
+    const int h = blockIdx.y, n = a.n;
+    const long nn = (long)n * n;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!a.update_A || q >= nn) return;
+    const int i = (int)(q % n), k = (int)(q / n);
+    const double* A = a.A + (size_t)h * nn;
+    const double* B = a.B + (size_t)h * nn + (long)k * n;
+    double s = 0.0;
+    for (int l = 0; l < n; ++l) s += A[i + (long)l * n] * B[l];
+    a.work[8 + (size_t)h * nn + q] = s;
+}
+
+/* A += +-lrate A B, reparameterisation, mu shift with the old W, c = cnew. */
+__global__ __launch_bounds__(1024) void mstep_c_kernel(MstepArgs a) {
     const int h = blockIdx.x, n = a.n, m = a.m;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const long nm = (long)n * m;
     const long nn = (long)n * n;
     double* A = a.A + (size_t)h * nn;
@@ -298,28 +317,20 @@ __global__ __launch_bounds__(kBlk) void mstep_c_kernel(MstepArgs a) {
     double* beta = a.beta + (size_t)h * nm;
     const double* cnw = a.cnew + (size_t)h * n;
     const double* W = a.W + (size_t)h * nn;
-    double* AB = a.work + 8 + (size_t)h * nn;
+    const double* AB = a.work + 8 + (size_t)h * nn;
     if (a.update_A) {
         const double lr = a.ds->lrate;
         const double f = a.newton[h] ? lr : -lr;
-        const double* B = a.B + (size_t)h * nn;
-        for (long q = threadIdx.x; q < nn; q += blockDim.x) {
-            const int i = (int)(q % n), k = (int)(q / n);
-            double s = 0.0;
-            for (int l = 0; l < n; ++l) s += A[i + (long)l * n] * B[l + (long)k * n];
-            AB[q] = s;
-        }
-        __syncthreads();
         for (long q = threadIdx.x; q < nn; q += blockDim.x) A[q] += f * AB[q];
         __syncthreads();
         if (a.do_reparm) {
-            /* normalizeAsave + normalizemubeta */
-            for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            /* normalizeAsave + normalizemubeta: one wave per column */
+            for (int k = wv; k < n; k += nw) {
                 double s = 0.0;
-                for (int i = 0; i < n; ++i) s += A[i + (long)k * n] * A[i + (long)k * n];
-                const double nu = sqrt(s);
-                for (int i = 0; i < n; ++i) A[i + (long)k * n] /= nu;
-                for (int j = 0; j < m; ++j) {
+                for (int i = lane; i < n; i += 64) s += A[i + (long)k * n] * A[i + (long)k * n];
+                const double nu = sqrt(wave_sum_f64(s));
+                for (int i = lane; i < n; i += 64) A[i + (long)k * n] /= nu;
+                for (int j = lane; j < m; j += 64) {
                     mu[(long)j * n + k] *= nu;
                     beta[(long)j * n + k] = beta[(long)j * n + k] / (nu * nu);
                 }
@@ -341,37 +352,48 @@ __global__ __launch_bounds__(kBlk) void mstep_c_kernel(MstepArgs a) {
 /* prepare: W = A^-1, ldet, GEMM copies of W, W c, E-step constants          */
 /* ------------------------------------------------------------------------ */
 
-__global__ __launch_bounds__(kBlk) void prepare_kernel(MstepArgs a) {
-    __shared__ int perm[1024];
-    const int h = blockIdx.x, n = a.n, m = a.m, NP = a.NP;
+/* bf16 hi and lo parts of one W entry, as the split-BF16 unmixing GEMM
+ * multiplies by them */
+__device__ __forceinline__ void split_w(double w, uint16_t& hi, uint16_t& lo) {
+    hi = f32_to_bf16_bits((float)w);
+    lo = f32_to_bf16_bits((float)(w - (double)bf16_bits_to_f32(hi)));
+}
+
+__device__ __forceinline__ double joined_w(uint16_t hi, uint16_t lo) {
+    return (double)bf16_bits_to_f32(hi) + (double)bf16_bits_to_f32(lo);
+}
+
+/* Everything after W = A^-1: the GEMM copies of W (rows h NP + i, columns
+ * k), W~ c with the W~ the unmixing GEMM multiplies by, the E-step
+ * constants and log|det W|.  Wl is W (column-major) in LDS or global. */
+__device__ void prepare_tail(const MstepArgs& a, int h, const double* Wl, double lad) {
+    const int n = a.n, m = a.m, NP = a.NP, tid = threadIdx.x;
     const long nn = (long)n * n, nm = (long)n * m;
-    const int tid = threadIdx.x;
     double* W = a.W + (size_t)h * nn;
-    double* X = a.work + 8 + (size_t)a.M * nn + (size_t)h * nn;
-    const double lad = block_lu_inverse(a.A + (size_t)h * nn, n, a.lu + (size_t)h * nn, X, W, perm);
-    /* GEMM copy of W (rows h*NP + i, columns k) and W~ c with the W~ the
-     * unmixing GEMM multiplies by: FP64 W, or bf16 hi + lo parts of W */
     for (long q = tid; q < nn; q += blockDim.x) {
-        const int i = (int)(q / n), k = (int)(q % n);
-        const double wv2 = W[i + (long)k * n];
+        const int i = (int)(q % n), k = (int)(q / n);
+        const double wv2 = Wl[q];
+        W[q] = wv2;
         const size_t dst = ((size_t)h * NP + i) * NP + k;
         if (a.prec_bf16) {
-            const uint16_t hi = f32_to_bf16_bits((float)wv2);
+            uint16_t hi, lo;
+            split_w(wv2, hi, lo);
             a.Whi[dst] = hi;
-            a.Wlo[dst] = f32_to_bf16_bits((float)(wv2 - (double)bf16_bits_to_f32(hi)));
+            a.Wlo[dst] = lo;
         } else {
             a.Wg64[dst] = wv2;
         }
     }
-    __syncthreads();
     const double* c = a.c + (size_t)h * n;
     for (int i = tid; i < n; i += blockDim.x) {
         double s = 0.0;
-        const size_t row = ((size_t)h * NP + i) * NP;
         for (int k = 0; k < n; ++k) {
-            const double wr = a.prec_bf16 ? (double)bf16_bits_to_f32(a.Whi[row + k]) +
-                                                (double)bf16_bits_to_f32(a.Wlo[row + k])
-                                          : a.Wg64[row + k];
+            double wr = Wl[i + (long)k * n];
+            if (a.prec_bf16) {
+                uint16_t hi, lo;
+                split_w(wr, hi, lo);
+                wr = joined_w(hi, lo);
+            }
             s += wr * c[k];
         }
         if (a.prec_bf16) a.wc[(size_t)h * NP + i] = (float)s;
@@ -404,6 +426,33 @@ __global__ __launch_bounds__(kBlk) void prepare_kernel(MstepArgs a) {
     }
 }
 
+/* n <= kLdsInvMax: W = A^-1 by Gauss-Jordan in LDS, one workgroup per model. */
+constexpr int kLdsInvMax = 128;
+
+__global__ __launch_bounds__(512) void prepare_lds_kernel(MstepArgs a) {
+    extern __shared__ double sm[];
+    __shared__ int swp[kLdsInvMax];
+    const int h = blockIdx.x, n = a.n;
+    const long nn = (long)n * n;
+    double* aw = sm;
+    const double* A = a.A + (size_t)h * nn;
+    for (long q = threadIdx.x; q < nn; q += blockDim.x) aw[q] = A[q];
+    __syncthreads();
+    const double lad = block_gj_inverse_lds(aw, n, sm + nn, sm + nn + n, swp);
+    prepare_tail(a, h, aw, lad);
+}
+
+/* larger n: LU with partial pivoting in global memory */
+__global__ __launch_bounds__(kBlk) void prepare_kernel(MstepArgs a) {
+    __shared__ int perm[1024];
+    const int h = blockIdx.x, n = a.n;
+    const long nn = (long)n * n;
+    double* W = a.W + (size_t)h * nn;
+    double* X = a.work + 8 + (size_t)a.M * nn + (size_t)h * nn;
+    const double lad = block_lu_inverse(a.A + (size_t)h * nn, n, a.lu + (size_t)h * nn, X, W, perm);
+    prepare_tail(a, h, W, lad);
+}
+
 }  // namespace
 
 int launch_reduce(const double* part, int nparts, size_t pstride, const double* gxpart,
@@ -423,13 +472,27 @@ int launch_reduce(const double* part, int nparts, size_t pstride, const double* 
 int launch_mstep(const MstepArgs& ma, hipStream_t st) {
     hipLaunchKernelGGL(mstep_a_kernel, dim3(ma.M), dim3(kBlk), 0, st, ma);
     hipLaunchKernelGGL(mstep_b_kernel, dim3(1), dim3(64), 0, st, ma);
-    hipLaunchKernelGGL(mstep_c_kernel, dim3(ma.M), dim3(kBlk), 0, st, ma);
+    const long nn = (long)ma.n * ma.n;
+    hipLaunchKernelGGL(ab_kernel, dim3((unsigned)((nn + 255) / 256), ma.M), dim3(256), 0, st, ma);
+    hipLaunchKernelGGL(mstep_c_kernel, dim3(ma.M), dim3(1024), 0, st, ma);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+static size_t prepare_lds_bytes(int n) { return ((size_t)n * n + 2 * (size_t)n) * sizeof(double); }
+
+int mstep_configure(int n) {
+    if (n > kLdsInvMax) return 0;
+    return hipFuncSetAttribute((const void*)prepare_lds_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)prepare_lds_bytes(n)) == hipSuccess ? 0 : -1;
 }
 
 int launch_prepare(const MstepArgs& ma, hipStream_t st) {
     if (ma.n > 1024) return -1;
-    hipLaunchKernelGGL(prepare_kernel, dim3(ma.M), dim3(kBlk), 0, st, ma);
+    if (ma.n <= kLdsInvMax)
+        hipLaunchKernelGGL(prepare_lds_kernel, dim3(ma.M), dim3(512), prepare_lds_bytes(ma.n), st, ma);
+    else
+        hipLaunchKernelGGL(prepare_kernel, dim3(ma.M), dim3(kBlk), 0, st, ma);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -254,7 +254,14 @@ __global__ __launch_bounds__(256) void sphere_diag_kernel(const double* __restri
 }
 
 /* x_white[t][i] = sum_k S[i][k] (x[t][k] - mean[k]) in FP64, stored float32
- * with the channel dimension padded to NP (zeros), plus an optional bf16 copy. */
+ * with the channel dimension padded to NP (zeros), plus optional split bf16
+ * copies: xbf[0 .. T NP) = bf16(x), xbf[T NP .. 2 T NP) = bf16(x - bf16(x)). */
+__device__ __forceinline__ void store_split_bf16(uint16_t* xbf, size_t lo_ofs, size_t e, float f) {
+    const uint16_t hi = f32_to_bf16_bits(f);
+    xbf[e] = hi;
+    xbf[lo_ofs + e] = f32_to_bf16_bits(f - bf16_bits_to_f32(hi));
+}
+
 __global__ __launch_bounds__(256) void whiten_kernel(const float* __restrict__ x,
                                                      const double* __restrict__ mean,
                                                      const double* __restrict__ S, long T, int n,
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(256) void whiten_kernel(const float* __restrict__ x
         const float f = (float)v;
         xw[(size_t)t * NP + i] = f;
         if (xw64) xw64[(size_t)t * NP + i] = v;
-        if (xbf) xbf[(size_t)t * NP + i] = f32_to_bf16_bits(f);
+        if (xbf) store_split_bf16(xbf, (size_t)T * NP, (size_t)t * NP + i, f);
     }
 }
 
@@ -294,7 +301,7 @@ __global__ void white_from_f64_kernel(const double* __restrict__ xin, long T, in
     const float f = (float)d;
     xw[e] = f;
     if (xw64) xw64[e] = d;
-    if (xbf) xbf[e] = f32_to_bf16_bits(f);
+    if (xbf) store_split_bf16(xbf, (size_t)T * NP, e, f);
 }
 
 __global__ void bf16_from_f32_kernel(const float* __restrict__ x, size_t count,
