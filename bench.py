@@ -81,6 +81,38 @@ def cpu_baseline(w, x, st0, precision, iters=2, target_s=15.0):
                       f"{dt:.1f} s, scaled by T/{Ts}"}, rel, Ts
 
 
+def copy_probe(mib):
+    """One hipMemset and one device-to-device hipMemcpy of `mib` MiB (the HIP
+    runtime's fill and copy kernels), so that a rocprofv3 --pmc pass over this
+    run can scale FETCH_SIZE and WRITE_SIZE by what they read for traffic of
+    known size."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed in the copy probe")
+    ok = (hip.hipMemset(a, 1, nbytes) == 0 and hip.hipMemcpy(b, a, nbytes, 3) == 0
+          and hip.hipDeviceSynchronize() == 0)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if not ok:
+        raise RuntimeError("copy probe failed")
+    log(f"copy probe: memset + copy of {mib} MiB")
+
+
+def measured_traffic(workload, prec):
+    """HBM bytes per E-step launch measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE passes (scaled by the copy probe), kept in profiles/ as JSON."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"traffic_{workload}_{'bf16' if prec == 16 else 'fp32'}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +122,8 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "bf16"], default=None)
     ap.add_argument("--T", type=int, default=None, help="override the sample count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run a memset + copy of MIB MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine does not split samples across GPUs "
@@ -103,6 +137,8 @@ def main():
     prec = w.precision if args.precision is None else (16 if args.precision == "bf16" else 32)
     n, T, M, m = w.n, w.T, w.M, w.m
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     t0 = time.perf_counter()
     x = ca.synth_data(w.spec())
     st0 = w.init_state()
@@ -150,7 +186,7 @@ def main():
         "config": {"workload": w.name, "n": n, "T": T, "M": M, "m": m,
                    "precision": "bf16" if prec == 16 else "fp32", "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(w.name, prec),
                      "kernel": "estep_kernel", "bytes_per_launch": nbytes,
                      "avg_ms": kms[1], "launches": cnt},
         "kernel_ms": {"prepare_unmix": kms[0], "estep": kms[1], "stat_gemm": kms[2],

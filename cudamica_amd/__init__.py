@@ -4,10 +4,11 @@ The EM iteration runs in hand-written HIP kernels for gfx950 behind the C
 ABI of include/amica_engine.h; `cudaamica -f <file>` (bin/) is the drop-in
 command line program with the reference's parameter-file format.
 """
-from .engine import (CLI_PATH, LIB_PATH, PREC_BF16, PREC_FP32, Engine, EngineError, cond,
-                     load_library, options, synth_data, synth_init, synth_spec)
+from .engine import (CLI_PATH, LIB_PATH, PARAM_NAMES, PREC_BF16, PREC_FP32, Engine, EngineError,
+                     cond, ll_adjoint, ll_forward, load_library, options, synth_data, synth_init,
+                     synth_spec)
 from .workloads import WORKLOADS, Workload
 
 __all__ = ["Engine", "EngineError", "options", "load_library", "synth_data", "synth_init",
            "synth_spec", "cond", "PREC_FP32", "PREC_BF16", "LIB_PATH", "CLI_PATH", "WORKLOADS",
-           "Workload"]
+           "Workload", "ll_forward", "ll_adjoint", "PARAM_NAMES"]

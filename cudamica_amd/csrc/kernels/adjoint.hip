@@ -1,0 +1,525 @@
+/*
+ * adjoint.hip -- the per-sample AMICA log-likelihood and its adjoint
+ * (include/amica_adjoint.h).
+ *
+ * The forward terms are the reference's E-step quantities
+ * (fraimondo/cudamica src/kernels.cu:251-450: y, the generalized-Gaussian
+ * log density Q, the log-sum-exp over components and over models).  The
+ * adjoint weights each sample's model posterior v_ht by the upstream
+ * gradient dll_t and re-evaluates the responsibilities u = dll v z (nothing
+ * is stored between the passes); the gradients are then the reference's
+ * sufficient statistics of those weighted responsibilities:
+ *   dalpha = sum u / alpha                  dmu  = sum sqrt(beta) u f
+ *   dbeta  = sum u (1 - f y) / (2 beta)     drho = sum u (psi(1+1/rho)/rho^2 - |y|^rho ln|y|)
+ *   dgm    = sum dll v / gm                 g    = sum_j sqrt(beta) u f   (per sample)
+ *   dW_h   = (sum_t dll v_h) W_h^-T - sum_t g_t (x_t - c_h)^T
+ *   dc_h   = W_h^T sum_t g_t
+ * with f = rho sign(y)|y|^(rho-1).
+ *
+ * Kernels: prep (per model: W^-1 by LU, log|det W|, W c, the constant part
+ * of Q), ll (one workgroup of ceil(n/64) waves per model over a range of
+ * samples; x tiles in LDS; b = W x - W c by each lane for its channel),
+ * gx (sum_t g_t x_t^T, 64 x 64 FP64 tiles over sample chunks), and finish
+ * (fixed-order sums over workgroups and the assembly of the gradients).
+ * Every reduction is in a fixed order, so results repeat bit for bit.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../../include/amica_adjoint.h"
+#include "../../../include/amica_engine.h"
+#include "common.h"
+#include "engine_internal.h"
+#include "linalg_dev.h"
+
+namespace amica {
+
+namespace {
+
+struct AdjArgs {
+    const void* x;
+    long T;
+    int n, NP, M, m;
+    const double *W, *c, *alpha, *mu, *beta, *rho, *gm;
+    double ldetS;
+    const double* dll;   /* null: forward only */
+    double* ll;          /* [T] or null */
+    /* workspace */
+    double* Winv;        /* [M][n][n] column-major */
+    double* ldet;        /* [M] log|det W_h| */
+    double* cq;          /* [M][m][n] log alpha + .5 log beta - log2 - lgamma(1+1/rho) */
+    double* wc;          /* [M][NP] W_h c_h */
+    double* lu;          /* [M][n][n] LU scratch */
+    double* X;           /* [M][n][n] inverse scratch */
+    void* g;             /* [T][M NP] (adjoint) */
+    double* part;        /* [nblocks][pstride] */
+    long chunk;          /* samples per workgroup */
+    size_t pstride;
+};
+
+/* per pair (h, i): dalpha_j, dmu_j, dbeta_j, drho_j (j < m), gsum; then Omega_h */
+__host__ __device__ inline int pair_stride(int m) { return 4 * m + 1; }
+
+__global__ __launch_bounds__(256) void adj_prep_kernel(AdjArgs a) {
+    __shared__ int perm[1024];
+    const int h = blockIdx.x, n = a.n, m = a.m;
+    const long nn = (long)n * n;
+    const double* W = a.W + (size_t)h * nn;
+    const double lad = block_lu_inverse(W, n, a.lu + (size_t)h * nn, a.X + (size_t)h * nn,
+                                        a.Winv + (size_t)h * nn, perm);
+    if (threadIdx.x == 0) a.ldet[h] = lad;
+    const double* c = a.c + (size_t)h * n;
+    for (int i = threadIdx.x; i < a.NP; i += blockDim.x) {
+        double s = 0.0;
+        if (i < n)
+            for (int k = 0; k < n; ++k) s += W[i + (long)k * n] * c[k];
+        a.wc[(size_t)h * a.NP + i] = s;
+    }
+    const long nm = (long)n * m;
+    for (long q = threadIdx.x; q < nm; q += blockDim.x) {
+        const size_t e = (size_t)h * nm + q;
+        a.cq[e] = log(a.alpha[e]) + 0.5 * log(a.beta[e]) - AMICA_LOG2_REF - lgamma(1.0 + 1.0 / a.rho[e]);
+    }
+}
+
+template <typename Real> __device__ __forceinline__ Real r_log(Real v);
+template <> __device__ __forceinline__ float r_log<float>(float v) { return logf(v); }
+template <> __device__ __forceinline__ double r_log<double>(double v) { return log(v); }
+template <typename Real> __device__ __forceinline__ Real r_exp(Real v);
+template <> __device__ __forceinline__ float r_exp<float>(float v) { return expf(v); }
+template <> __device__ __forceinline__ double r_exp<double>(double v) { return exp(v); }
+
+template <typename Real> struct RealTiny;
+template <> struct RealTiny<float> { static constexpr float v = 1.17549435e-38f; };
+template <> struct RealTiny<double> { static constexpr double v = 2.2250738585072014e-308; };
+
+/* Samples per tile held in LDS and registers. */
+template <typename Real> struct Tile { static constexpr int S = sizeof(Real) == 8 ? 4 : 8; };
+
+/* Component j of channel i of model h at sample value b: y, Q, and (for the
+ * adjoint) f y, f and |y|^rho ln|y|. */
+template <typename Real>
+__device__ __forceinline__ void density(Real b, Real sb, Real mu, Real rho, Real cq, Real& y,
+                                        Real& Q, Real& yr, Real& lny) {
+    y = sb * (b - mu);
+    const Real ay = fmax(fabs(y), RealTiny<Real>::v);
+    lny = r_log<Real>(ay);
+    yr = r_exp<Real>(rho * lny);     /* |y|^rho */
+    Q = cq - yr;
+}
+
+template <typename Real, int MJ, bool ADJ>
+__global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
+    constexpr int S = Tile<Real>::S;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = a.n, NP = a.NP, M = a.M;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+    const int CW = NP / 64;
+    const int h = wv / CW;
+    const int i = (wv - h * CW) * 64 + lane;
+    const bool act = i < n;
+    /* LDS: xs[S][NP] (Real), qp[S][nw], Lh[S][M], vh[S][M] (double) */
+    Real* xs = (Real*)smem_raw;
+    double* qp = (double*)(smem_raw + (size_t)S * NP * sizeof(Real));
+    double* Lh = qp + S * nw;
+    double* vh = Lh + S * M;
+
+    Real sb[MJ], mu[MJ], rho[MJ], cq[MJ];
+    double psi1[ADJ ? MJ : 1], al[ADJ ? MJ : 1], be[ADJ ? MJ : 1];
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) {
+        const size_t e = ((size_t)h * MJ + j) * n + (act ? i : 0);
+        sb[j] = (Real)sqrt(a.beta[e]);
+        mu[j] = (Real)a.mu[e];
+        rho[j] = (Real)a.rho[e];
+        cq[j] = (Real)a.cq[e];
+        if constexpr (ADJ) {
+            psi1[j] = psi_dev(1.0 + 1.0 / a.rho[e]);
+            al[j] = a.alpha[e];
+            be[j] = a.beta[e];
+        }
+    }
+    const Real wci = (Real)a.wc[(size_t)h * NP + i];
+    const double* Wh = a.W + (size_t)h * n * n;
+    double ga[ADJ ? MJ : 1], gmu[ADJ ? MJ : 1], gb[ADJ ? MJ : 1], gr[ADJ ? MJ : 1];
+    double gsum = 0.0, om = 0.0;
+#pragma unroll
+    for (int j = 0; j < (ADJ ? MJ : 1); ++j) ga[j] = gmu[j] = gb[j] = gr[j] = 0.0;
+
+    const long t0 = (long)blockIdx.x * a.chunk;
+    const long t1 = min(a.T, t0 + a.chunk);
+    const Real* x = (const Real*)a.x;
+    for (long tb = t0; tb < t1; tb += S) {
+        for (int q = tid; q < S * NP; q += blockDim.x) {
+            const int s = q / NP, k = q - (q / NP) * NP;
+            const long t = tb + s;
+            xs[q] = (t < t1 && k < n) ? x[(size_t)t * n + k] : (Real)0;
+        }
+        __syncthreads();
+        Real bv[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) bv[s] = (Real)0;
+        if (act)
+            for (int k = 0; k < n; ++k) {
+                const Real wik = (Real)Wh[i + (long)k * n];
+#pragma unroll
+                for (int s = 0; s < S; ++s) bv[s] = fma(wik, xs[s * NP + k], bv[s]);
+            }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            bv[s] -= wci;
+            double q = 0.0;
+            if (act) {
+                Real Qj[MJ], mx = (Real)0;
+#pragma unroll
+                for (int j = 0; j < MJ; ++j) {
+                    Real y, yr, lny;
+                    density<Real>(bv[s], sb[j], mu[j], rho[j], cq[j], y, Qj[j], yr, lny);
+                    mx = (j == 0) ? Qj[j] : fmax(mx, Qj[j]);
+                }
+                Real se = (Real)0;
+#pragma unroll
+                for (int j = 0; j < MJ; ++j) se += r_exp<Real>(Qj[j] - mx);
+                q = (double)mx + (double)r_log<Real>(se);
+            }
+            q = wave_sum_f64(q);
+            if (lane == 0) qp[s * nw + wv] = q;
+        }
+        __syncthreads();
+        if (tid < S * M) {
+            const int s = tid / M, hh = tid - (tid / M) * M;
+            double L = log(a.gm[hh]) + a.ldet[hh] + a.ldetS;
+            for (int c = 0; c < CW; ++c) L += qp[s * nw + hh * CW + c];
+            Lh[s * M + hh] = L;
+        }
+        __syncthreads();
+        if (tid < S) {
+            const int s = tid;
+            const long t = tb + s;
+            double mx = Lh[s * M];
+            for (int hh = 1; hh < M; ++hh) mx = fmax(mx, Lh[s * M + hh]);
+            double se = 0.0;
+            for (int hh = 0; hh < M; ++hh) se += exp(Lh[s * M + hh] - mx);
+            const double P = mx + log(se);
+            const bool valid = t < t1;
+            if (valid && a.ll) a.ll[t] = P;
+            const double w = (ADJ && valid) ? a.dll[t] : 0.0;
+            for (int hh = 0; hh < M; ++hh) vh[s * M + hh] = w * exp(Lh[s * M + hh] - P);
+        }
+        __syncthreads();
+        if constexpr (ADJ) {
+            Real* g = (Real*)a.g;
+            const size_t ldg = (size_t)M * NP;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const long t = tb + s;
+                if (t >= t1) break;
+                const double omh = vh[s * M + h];
+                double gv = 0.0;
+                if (act) {
+                    Real Qj[MJ], Yj[MJ], YRj[MJ], LNj[MJ], mx = (Real)0;
+#pragma unroll
+                    for (int j = 0; j < MJ; ++j) {
+                        density<Real>(bv[s], sb[j], mu[j], rho[j], cq[j], Yj[j], Qj[j], YRj[j], LNj[j]);
+                        mx = (j == 0) ? Qj[j] : fmax(mx, Qj[j]);
+                    }
+                    Real ez[MJ], se = (Real)0;
+#pragma unroll
+                    for (int j = 0; j < MJ; ++j) {
+                        ez[j] = r_exp<Real>(Qj[j] - mx);
+                        se += ez[j];
+                    }
+#pragma unroll
+                    for (int j = 0; j < MJ; ++j) {
+                        const double u = omh * (double)(ez[j] / se);
+                        /* f y = rho |y|^rho and f = rho |y|^rho / y (y != 0) */
+                        const double fy = (double)rho[j] * (double)YRj[j];
+                        const double ay = fmax(fabs((double)Yj[j]), (double)RealTiny<Real>::v);
+                        const double f = (Yj[j] == (Real)0) ? 0.0 : copysign(fy / ay, (double)Yj[j]);
+                        const double sbj = sqrt(be[j]);
+                        ga[j] += u / al[j];
+                        gmu[j] += sbj * u * f;
+                        gb[j] += u * (1.0 - fy) / (2.0 * be[j]);
+                        gr[j] += u * (psi1[j] / ((double)rho[j] * (double)rho[j]) -
+                                      (double)YRj[j] * (double)LNj[j]);
+                        gv += sbj * u * f;
+                    }
+                }
+                g[(size_t)t * ldg + (size_t)h * NP + i] = (Real)gv;
+                gsum += gv;
+                if (i == 0) om += omh;
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (ADJ) {
+        double* base = a.part + (size_t)blockIdx.x * a.pstride;
+        double* pp = base + ((size_t)h * NP + i) * pair_stride(MJ);
+#pragma unroll
+        for (int j = 0; j < MJ; ++j) {
+            pp[j] = ga[j];
+            pp[MJ + j] = gmu[j];
+            pp[2 * MJ + j] = gb[j];
+            pp[3 * MJ + j] = gr[j];
+        }
+        pp[4 * MJ] = gsum;
+        if (i == 0) base[(size_t)M * NP * pair_stride(MJ) + h] = om;
+    }
+}
+
+/* part[chunk][r][k] = sum_{t in chunk} g[t][r] x[t][k]   (r < M NP, k < n) */
+template <typename Real>
+__global__ __launch_bounds__(256) void adj_gx_kernel(const Real* __restrict__ g,
+                                                     const Real* __restrict__ x, long T, int n,
+                                                     int ldg, long chunk, double* __restrict__ part) {
+    __shared__ double ga[32][64];
+    __shared__ double xb[32][64];
+    const int tid = threadIdx.x;
+    const int rb = blockIdx.y * 64, kb = blockIdx.z * 64;
+    const long ts = (long)blockIdx.x * chunk, te = min(T, ts + chunk);
+    const int ti = (tid & 15) * 4, tk = (tid >> 4) * 4;
+    double acc[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
+    for (long t0 = ts; t0 < te; t0 += 32) {
+        for (int q = tid; q < 32 * 64; q += 256) {
+            const int s = q >> 6, col = q & 63;
+            const long t = t0 + s;
+            double va = 0.0, vb = 0.0;
+            if (t < te) {
+                if (rb + col < ldg) va = (double)g[(size_t)t * ldg + rb + col];
+                if (kb + col < n) vb = (double)x[(size_t)t * n + kb + col];
+            }
+            ga[s][col] = va;
+            xb[s][col] = vb;
+        }
+        __syncthreads();
+        for (int s = 0; s < 32; ++s) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) av[p] = ga[s][ti + p];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) bv[q] = xb[s][tk + q];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] = fma(av[p], bv[q], acc[p][q]);
+        }
+        __syncthreads();
+    }
+    double* out = part + (size_t)blockIdx.x * ldg * n;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = rb + ti + p, k = kb + tk + q;
+            if (r < ldg && k < n) out[(size_t)r * n + k] = acc[p][q];
+        }
+}
+
+struct AdjOut {
+    double *dW, *dc, *dalpha, *dmu, *dbeta, *drho, *dgm;
+};
+
+/* One workgroup per model: fixed-order sums over the ll workgroups and the
+ * gx chunks, then the gradients. */
+__global__ __launch_bounds__(256) void adj_finish_kernel(AdjArgs a, int nblocks,
+                                                         const double* __restrict__ gxpart,
+                                                         int nchunks, AdjOut o) {
+    extern __shared__ double gs[];   /* gsum_h[n] */
+    __shared__ double om_sh;
+    const int h = blockIdx.x, n = a.n, m = a.m, NP = a.NP, M = a.M;
+    const int ps = pair_stride(m);
+    const long nm = (long)n * m;
+    for (long q = threadIdx.x; q < nm + n; q += blockDim.x) {
+        /* q < nm: (j, i) of the four per-component gradients; then gsum_i */
+        const int j = q < nm ? (int)(q / n) : 0;
+        const int i = q < nm ? (int)(q % n) : (int)(q - nm);
+        const size_t off = ((size_t)h * NP + i) * ps;
+        if (q < nm) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            for (int b = 0; b < nblocks; ++b) {
+                const double* pp = a.part + (size_t)b * a.pstride + off;
+                s0 += pp[j];
+                s1 += pp[m + j];
+                s2 += pp[2 * m + j];
+                s3 += pp[3 * m + j];
+            }
+            const size_t e = (size_t)h * nm + q;
+            o.dalpha[e] = s0;
+            o.dmu[e] = s1;
+            o.dbeta[e] = s2;
+            o.drho[e] = s3;
+        } else {
+            double s = 0.0;
+            for (int b = 0; b < nblocks; ++b) s += a.part[(size_t)b * a.pstride + off + 4 * m];
+            gs[i] = s;
+        }
+    }
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nblocks; ++b) s += a.part[(size_t)b * a.pstride + (size_t)M * NP * ps + h];
+        om_sh = s;
+        o.dgm[h] = s / a.gm[h];
+    }
+    __syncthreads();
+    const double om = om_sh;
+    const long nn = (long)n * n;
+    const double* W = a.W + (size_t)h * nn;
+    const double* Wi = a.Winv + (size_t)h * nn;
+    const double* c = a.c + (size_t)h * n;
+    const size_t ldg = (size_t)M * NP;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += W[i + (long)k * n] * gs[i];
+        o.dc[(size_t)h * n + k] = s;
+    }
+    for (long q = threadIdx.x; q < nn; q += blockDim.x) {
+        const int i = (int)(q % n), k = (int)(q / n);
+        double G = 0.0;
+        for (int ch = 0; ch < nchunks; ++ch) G += gxpart[((size_t)ch * ldg + (size_t)h * NP + i) * n + k];
+        o.dW[(size_t)h * nn + q] = om * Wi[k + (long)i * n] - (G - gs[i] * c[k]);
+    }
+}
+
+template <typename Real, bool ADJ>
+int launch_ll(int MJ, const AdjArgs& a, int nblocks, size_t shm, hipStream_t st) {
+    dim3 grid(nblocks), block(64 * a.M * (a.NP / 64));
+#define AMICA_LL_CASE(J)                                                                           \
+    case J:                                                                                        \
+        if (hipFuncSetAttribute((const void*)ll_kernel<Real, J, ADJ>,                              \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) != hipSuccess) \
+            return -1;                                                                             \
+        hipLaunchKernelGGL((ll_kernel<Real, J, ADJ>), grid, block, shm, st, a);                   \
+        break;
+    switch (MJ) {
+        AMICA_LL_CASE(1)
+        AMICA_LL_CASE(2)
+        AMICA_LL_CASE(3)
+        AMICA_LL_CASE(4)
+        AMICA_LL_CASE(5)
+        AMICA_LL_CASE(6)
+        AMICA_LL_CASE(7)
+        AMICA_LL_CASE(8)
+        default: return -1;
+    }
+#undef AMICA_LL_CASE
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+/* Workspace of one call, freed on every exit path. */
+struct Workspace {
+    void* ptrs[8] = {nullptr};
+    int n = 0;
+    ~Workspace() {
+        for (int k = 0; k < n; ++k) (void)hipFree(ptrs[k]);
+    }
+    bool alloc(void** p, size_t bytes) {
+        if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+        ptrs[n++] = *p;
+        return true;
+    }
+};
+
+int ll_call(int real_bits, const void* x, long T, int n, int M, int m, const double* W,
+            const double* c, const double* alpha, const double* mu, const double* beta,
+            const double* rho, const double* gm, double ldetS, const double* dll, double* ll,
+            const AdjOut* out, void* stream) {
+    const bool adj = out != nullptr;
+    if (real_bits != 32 && real_bits != 64) return report_error(AMICA_E_INVALID, "real_bits must be 32 or 64");
+    if (!x || !W || !c || !alpha || !mu || !beta || !rho || !gm || (adj && !dll))
+        return report_error(AMICA_E_INVALID, "null argument");
+    if (adj && (!out->dW || !out->dc || !out->dalpha || !out->dmu || !out->dbeta || !out->drho || !out->dgm))
+        return report_error(AMICA_E_INVALID, "null gradient output");
+    if (!adj && !ll) return report_error(AMICA_E_INVALID, "null ll output");
+    if (T < 1 || n < 1 || M < 1 || m < 1 || m > 8)
+        return report_error(AMICA_E_INVALID, "need T >= 1, n >= 1, M >= 1, 1 <= m <= 8");
+    const int NP = (n + 63) / 64 * 64;
+    if ((long)M * NP > 512) return report_error(AMICA_E_INVALID, "M * ceil(n/64) * 64 must be <= 512");
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return report_error(AMICA_E_DEVICE, "no HIP device");
+
+    const size_t rb = real_bits == 64 ? 8 : 4;
+    const int S = real_bits == 64 ? 4 : 8;
+    const int nw = M * NP / 64;
+    const long nblocks = std::max(1L, std::min((long)cus * 2, (T + S - 1) / S));
+    long chunk = (T + nblocks - 1) / nblocks;
+    chunk = (chunk + S - 1) / S * S;
+    const int nb = (int)((T + chunk - 1) / chunk);
+
+    AdjArgs a = {};
+    a.x = x; a.T = T; a.n = n; a.NP = NP; a.M = M; a.m = m;
+    a.W = W; a.c = c; a.alpha = alpha; a.mu = mu; a.beta = beta; a.rho = rho; a.gm = gm;
+    a.ldetS = ldetS; a.dll = adj ? dll : nullptr; a.ll = ll; a.chunk = chunk;
+    a.pstride = ((size_t)M * NP * pair_stride(m) + M + 31) & ~(size_t)31;
+    Workspace ws;
+    const size_t nn = (size_t)n * n;
+    if (!ws.alloc((void**)&a.Winv, M * nn * 8) || !ws.alloc((void**)&a.ldet, M * 8) ||
+        !ws.alloc((void**)&a.cq, (size_t)M * m * n * 8) || !ws.alloc((void**)&a.wc, (size_t)M * NP * 8) ||
+        !ws.alloc((void**)&a.lu, M * nn * 8) || !ws.alloc((void**)&a.X, M * nn * 8))
+        return report_error(AMICA_E_NOMEM, "workspace allocation failed");
+    const int gchunks = (int)std::max(1L, std::min((long)cus, (T + 255) / 256));
+    double* gxpart = nullptr;
+    if (adj) {
+        if (!ws.alloc(&a.g, (size_t)T * M * NP * rb) ||
+            !ws.alloc((void**)&a.part, (size_t)nb * a.pstride * 8) ||
+            !ws.alloc((void**)&gxpart, (size_t)gchunks * M * NP * n * 8))
+            return report_error(AMICA_E_NOMEM, "workspace allocation failed");
+    }
+    hipLaunchKernelGGL(adj_prep_kernel, dim3(M), dim3(256), 0, st, a);
+    if (hipGetLastError() != hipSuccess) return report_error(AMICA_E_DEVICE, "prep launch failed");
+    const size_t shm = (size_t)S * NP * rb + (size_t)S * nw * 8 + 2 * (size_t)S * M * 8;
+    int rc;
+    if (real_bits == 64)
+        rc = adj ? launch_ll<double, true>(m, a, nb, shm, st) : launch_ll<double, false>(m, a, nb, shm, st);
+    else
+        rc = adj ? launch_ll<float, true>(m, a, nb, shm, st) : launch_ll<float, false>(m, a, nb, shm, st);
+    if (rc) return report_error(AMICA_E_DEVICE, "likelihood kernel launch failed");
+    if (adj) {
+        long gch = (T + gchunks - 1) / gchunks;
+        gch = (gch + 31) / 32 * 32;
+        const int gc = (int)((T + gch - 1) / gch);
+        const dim3 grid(gc, (M * NP + 63) / 64, (n + 63) / 64);
+        if (real_bits == 64)
+            hipLaunchKernelGGL(adj_gx_kernel<double>, grid, dim3(256), 0, st, (const double*)a.g,
+                               (const double*)x, T, n, M * NP, gch, gxpart);
+        else
+            hipLaunchKernelGGL(adj_gx_kernel<float>, grid, dim3(256), 0, st, (const float*)a.g,
+                               (const float*)x, T, n, M * NP, gch, gxpart);
+        hipLaunchKernelGGL(adj_finish_kernel, dim3(M), dim3(256), (size_t)n * 8, st, a, nb, gxpart, gc, *out);
+        if (hipGetLastError() != hipSuccess) return report_error(AMICA_E_DEVICE, "adjoint launch failed");
+    }
+    const hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return report_error(AMICA_E_DEVICE, hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+}  // namespace amica
+
+extern "C" int amica_ll_forward(int real_bits, const void* x, long T, int n, int M, int m,
+                                const double* W, const double* c, const double* alpha,
+                                const double* mu, const double* beta, const double* rho,
+                                const double* gm, double ldetS, double* ll, void* stream) {
+    return amica::ll_call(real_bits, x, T, n, M, m, W, c, alpha, mu, beta, rho, gm, ldetS, nullptr,
+                          ll, nullptr, stream);
+}
+
+extern "C" int amica_ll_adjoint(int real_bits, const void* x, long T, int n, int M, int m,
+                                const double* W, const double* c, const double* alpha,
+                                const double* mu, const double* beta, const double* rho,
+                                const double* gm, double ldetS, const double* dll, double* ll,
+                                double* dW, double* dc, double* dalpha, double* dmu, double* dbeta,
+                                double* drho, double* dgm, void* stream) {
+    const amica::AdjOut out = {dW, dc, dalpha, dmu, dbeta, drho, dgm};
+    return amica::ll_call(real_bits, x, T, n, M, m, W, c, alpha, mu, beta, rho, gm, ldetS, dll, ll,
+                          &out, stream);
+}

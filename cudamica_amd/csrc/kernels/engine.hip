@@ -40,6 +40,12 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+int amica::report_error(int code, const char* msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIPCHK(call)                                                                  \
     do {                                                                              \
         hipError_t _e = (call);                                                       \

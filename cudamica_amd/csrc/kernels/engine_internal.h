@@ -10,6 +10,9 @@
 
 namespace amica {
 
+/* Sets the message amica_last_error() returns; returns code. */
+int report_error(int code, const char* msg);
+
 /* Per-(h, j, i) FP32 constants of the E-step, table [M][kNumCst][m][NP]. */
 enum {
     kCstSb = 0,     /* sqrt(beta) */

@@ -1,6 +1,7 @@
 /*
  * linalg_dev.h -- workgroup-level FP64 dense linear algebra used by the
- * engine's small per-model kernels (n <= 1024, column-major).
+ * engine's small per-model kernels (n <= 1024, column-major), and the
+ * digamma function of the rho update and the likelihood adjoint.
  */
 #ifndef AMICA_LINALG_DEV_H_
 #define AMICA_LINALG_DEV_H_
@@ -8,6 +9,22 @@
 #include "common.h"
 
 namespace amica {
+
+/* Digamma by the Stirling asymptotic series (Bernoulli coefficients
+ * B_2k / 2k) at x + 6 and the recurrence psi(x) = psi(x+1) - 1/x. */
+__device__ inline double psi_dev(double x0) {
+    const double x = x0 + 6.0;
+    const double inv = 1.0 / (x * x);
+    const double coef[9] = {1.0 / 12.0, -1.0 / 120.0, 1.0 / 252.0, -1.0 / 240.0, 1.0 / 132.0,
+                            -691.0 / 32760.0, 1.0 / 12.0, -3617.0 / 8160.0, 43867.0 / 14364.0};
+    double r = log(x) - 0.5 / x;
+    double p = inv;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { r -= coef[k] * p; p *= inv; }
+#pragma unroll
+    for (int k = 5; k >= 0; --k) r -= 1.0 / (x0 + k);
+    return r;
+}
 
 /* Inverse of the n x n matrix A by LU with partial pivoting, computed by
  * one workgroup (blockDim.x a multiple of 64, <= 1024).  lu and X are n*n

@@ -54,22 +54,6 @@ __device__ double block_sum(double v, double* sh) {
     return r;
 }
 
-/* Digamma by the Stirling asymptotic series (Bernoulli coefficients
- * B_2k / 2k) at x + 6 and the recurrence psi(x) = psi(x+1) - 1/x. */
-__device__ double psi_dev(double x0) {
-    const double x = x0 + 6.0;
-    const double inv = 1.0 / (x * x);
-    const double coef[9] = {1.0 / 12.0, -1.0 / 120.0, 1.0 / 252.0, -1.0 / 240.0, 1.0 / 132.0,
-                            -691.0 / 32760.0, 1.0 / 12.0, -3617.0 / 8160.0, 43867.0 / 14364.0};
-    double r = log(x) - 0.5 / x;
-    double p = inv;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) { r -= coef[k] * p; p *= inv; }
-#pragma unroll
-    for (int k = 5; k >= 0; --k) r -= 1.0 / (x0 + k);
-    return r;
-}
-
 /* ------------------------------------------------------------------------ */
 /* reductions                                                                 */
 /* ------------------------------------------------------------------------ */

@@ -78,6 +78,10 @@ _SIGNATURES = [
     ("amica_synth_init", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
                                         ctypes.c_int, ctypes.c_double, _D, _D, _D, _D, _D, _D, _D]),
     ("amica_synth_cond", ctypes.c_double, [ctypes.c_int, _D]),
+    ("amica_ll_forward", ctypes.c_int, [ctypes.c_int, _V, ctypes.c_long, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int] + [_V] * 7 + [ctypes.c_double, _V, _V]),
+    ("amica_ll_adjoint", ctypes.c_int, [ctypes.c_int, _V, ctypes.c_long, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int] + [_V] * 7 + [ctypes.c_double] + [_V] * 10),
 ]
 
 _lib = None
@@ -310,3 +314,23 @@ def synth_init(n, M, m, seed, fix_init=1, rho0=1.5):
 def cond(A):
     A = np.ascontiguousarray(np.asarray(A, np.float64).T)  # column-major flat
     return load_library().amica_synth_cond(A.shape[0], _d(A.ravel()))
+
+
+# per-sample log-likelihood and its adjoint (include/amica_adjoint.h) -------
+PARAM_NAMES = ("W", "c", "alpha", "mu", "beta", "rho", "gm")
+
+
+def ll_forward(real_bits, x, T, n, M, m, params, ldetS, ll, stream=0):
+    """amica_ll_forward on device buffers given as integer addresses: x, ll
+    and params[name] for name in PARAM_NAMES (FP64 device arrays)."""
+    L = load_library()
+    _check(L.amica_ll_forward(real_bits, x, T, n, M, m, *[params[k] for k in PARAM_NAMES],
+                              ldetS, ll, stream), "amica_ll_forward")
+
+
+def ll_adjoint(real_bits, x, T, n, M, m, params, ldetS, dll, grads, ll=0, stream=0):
+    """amica_ll_adjoint on device addresses; grads[name] receives d/d name."""
+    L = load_library()
+    _check(L.amica_ll_adjoint(real_bits, x, T, n, M, m, *[params[k] for k in PARAM_NAMES], ldetS,
+                              dll, ll, *[grads[k] for k in PARAM_NAMES], stream),
+           "amica_ll_adjoint")

@@ -86,3 +86,17 @@ def test_init_state_recipe(built):
     assert np.all(st["beta"] == 1.0) and np.all(st["rho"] == 1.5)
     np.testing.assert_allclose(st["alpha"], 1.0 / m)
     np.testing.assert_allclose(st["gm"], 0.5)
+
+
+def test_ll_entry_points_validate_before_touching_gpu(built):
+    """amica_ll_forward / amica_ll_adjoint reject bad arguments with
+    AMICA_E_INVALID (-3) before any device call (dummy addresses are never
+    dereferenced)."""
+    L = ca.load_library()
+    p = [1] * 7
+    assert L.amica_ll_forward(16, 1, 10, 4, 1, 3, *p, 0.0, 1, None) == -3        # real_bits
+    assert L.amica_ll_forward(32, 1, 10, 4, 1, 9, *p, 0.0, 1, None) == -3        # m > 8
+    assert L.amica_ll_forward(32, 1, 10, 300, 2, 3, *p, 0.0, 1, None) == -3      # M NP > 512
+    assert L.amica_ll_forward(64, 1, 10, 4, 1, 3, *p, 0.0, None, None) == -3     # no ll output
+    assert b"ll output" in L.amica_last_error()
+    assert L.amica_ll_adjoint(64, 1, 10, 4, 1, 3, *p, 0.0, None, None, *p, None) == -3  # no dll
