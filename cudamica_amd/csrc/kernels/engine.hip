@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "../../../include/amica_batch.h"
 #include "../../../include/amica_engine.h"
 #include "common.h"
 #include "engine_internal.h"
@@ -83,6 +84,9 @@ struct amica_engine {
     void* g = nullptr;
     /* partials */
     int nblocks = 0;
+    int nblocks_cap = 0;           /* E-step partial slots allocated */
+    int nchunks_cap = 0;           /* statistic-GEMM partial slots allocated */
+    int cu_budget = 0;             /* CUs the grids are sized for (default: all) */
     int estep_wg_per_cu = 0;
     long chunk = 0;
     size_t pstride = 0;
@@ -124,6 +128,32 @@ struct amica_engine {
     size_t ev_used = 0;
     hipEvent_t* ev = nullptr;
 
+    /* Grid sizes for a budget of `budget` CUs: the E-step gets budget x
+     * (resident workgroups per CU) workgroups of whole 32-sample chunks, the
+     * statistic GEMM its chunk count; both stay within the partial slots
+     * allocated for the whole device. */
+    void size_grids(int budget) {
+        cu_budget = std::max(1, std::min(budget, cus));
+        const long T = o.T;
+        const long target = (long)cu_budget * estep_wg_per_cu;
+        nblocks = (int)std::max(1L, std::min(target, (T + 31) / 32));
+        chunk = (T + nblocks - 1) / nblocks;
+        chunk = ((chunk + 31) / 32) * 32;     /* whole packed 8-sample groups */
+        nblocks = (int)((T + chunk - 1) / chunk);
+        if (bf16) {
+            nchunks = gx_bf16_chunks(T, NP, cu_budget);
+        } else {
+            const int tiles = (ldb / 64) * (NP / 64);
+            nchunks = (int)std::max(1L, std::min((long)(2 * cu_budget + tiles - 1) / tiles, (T + 31) / 32));
+        }
+        if (gexec) {   /* the captured grids are stale */
+            (void)hipGraphExecDestroy(gexec);
+            (void)hipGraphDestroy(graph);
+            gexec = nullptr;
+            graph = nullptr;
+        }
+    }
+
     MstepArgs margs() const {
         MstepArgs a;
         std::memset(&a, 0, sizeof(a));
@@ -164,7 +194,7 @@ struct amica_engine {
         if (with_timing) HIPCHK(hipEventRecord(ev[0], st));
         LCHK(launch_prepare(ma, st));
         if (bf16)
-            LCHK(launch_unmix_bf16x3(xw, xbf, Whi, Wlo, wc, b, o.T, NP, ldb, cus, st));
+            LCHK(launch_unmix_bf16x3(xw, xbf, Whi, Wlo, wc, b, o.T, NP, ldb, cu_budget, st));
         else
             LCHK(launch_unmix_f64(xw64, Wg64, wc64, b64, o.T, NP, ldb, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
@@ -288,29 +318,21 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
         amica_engine_destroy(e);
         return fail(AMICA_E_DEVICE, "M-step LDS configuration failed");
     }
-    const int wg_per_cu = estep_blocks_per_cu(m, !e->bf16, 64 * waves);
+    const int wg_per_cu = estep_blocks_per_cu(m, !e->bf16, 64 * waves, M == 1);
     if (wg_per_cu < 1) {
         amica_engine_destroy(e);
         return fail(AMICA_E_DEVICE, "E-step occupancy query failed");
     }
     e->estep_wg_per_cu = wg_per_cu;
-    long target = (long)e->cus * wg_per_cu;
-    long maxb = (T + 31) / 32;
-    e->nblocks = (int)std::max(1L, std::min(target, maxb));
-    e->chunk = (T + e->nblocks - 1) / e->nblocks;
-    e->chunk = ((e->chunk + 31) / 32) * 32;     /* whole packed 8-sample groups */
-    e->nblocks = (int)((T + e->chunk - 1) / e->chunk);
+    /* partial slots for the largest grids (the whole device) */
+    e->size_grids(e->cus);
+    e->nblocks_cap = e->nblocks;
+    e->nchunks_cap = e->nchunks;
     const int NS = kStatsPerJ * m + kStatsPerI;
     e->pstride = (size_t)M * NS * NP + M + 1;
     e->pstride = (e->pstride + 31) & ~(size_t)31;
-    ALLOC(e->part, (size_t)e->nblocks * e->pstride * sizeof(double));
-    if (e->bf16) {
-        e->nchunks = gx_bf16_chunks(T, NP, e->cus);
-    } else {
-        const int tiles = (e->ldb / 64) * (NP / 64);
-        e->nchunks = (int)std::max(1L, std::min((long)(2 * e->cus + tiles - 1) / tiles, (T + 31) / 32));
-    }
-    ALLOC(e->gxpart, (size_t)e->nchunks * e->ldb * NP * sizeof(double));
+    ALLOC(e->part, (size_t)e->nblocks_cap * e->pstride * sizeof(double));
+    ALLOC(e->gxpart, (size_t)e->nchunks_cap * e->ldb * NP * sizeof(double));
     ALLOC(e->raw, e->pstride * sizeof(double));
     ALLOC(e->gxtot, (size_t)e->ldb * NP * sizeof(double));
     const size_t nn = (size_t)n * n, nm = (size_t)n * m;
@@ -737,6 +759,240 @@ int amica_engine_get_timing(amica_engine* e, double* ms, int* count) {
     for (int k = 0; k < 4; ++k) ms[k] = e->ev_used ? sum[k] / (double)e->ev_used : 0.0;
     if (count) *count = (int)e->ev_used;
     return 0;
+}
+
+}  // extern "C"
+
+/* ------------------------------------------------------------------------ */
+/* batches of recordings (include/amica_batch.h)                              */
+/* ------------------------------------------------------------------------ */
+
+struct amica_batch {
+    std::vector<amica_engine*> eng;
+    int device = 0;
+    int cus = 0;
+    hipStream_t cap = nullptr;          /* the stream the batch graph is launched on */
+    hipEvent_t fork = nullptr;
+    hipEvent_t done = nullptr;          /* end of the last launch; engine streams wait on it */
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+
+    /* one iteration of every recording: fork from `cap`, each recording on
+     * its own stream, join back into `cap` */
+    int capture() {
+        HIPCHK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        int rc = hipEventRecord(fork, cap) == hipSuccess ? 0 : AMICA_E_DEVICE;
+        for (size_t r = 0; r < eng.size() && rc == 0; ++r) {
+            if (hipStreamWaitEvent(eng[r]->st, fork, 0) != hipSuccess) { rc = AMICA_E_DEVICE; break; }
+            rc = eng[r]->enqueue_iteration(false);
+            if (rc) break;
+            if (hipEventRecord(join[r], eng[r]->st) != hipSuccess ||
+                hipStreamWaitEvent(cap, join[r], 0) != hipSuccess)
+                rc = AMICA_E_DEVICE;
+        }
+        hipGraph_t gr = nullptr;
+        const hipError_t ce = hipStreamEndCapture(cap, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc == AMICA_E_DEVICE ? fail(AMICA_E_DEVICE, "batch capture failed") : rc;
+        }
+        if (ce != hipSuccess) return fail(AMICA_E_DEVICE, std::string("batch capture: ") + hipGetErrorString(ce));
+        graph = gr;
+        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        return 0;
+    }
+};
+
+extern "C" {
+
+int amica_batch_destroy(amica_batch* b) {
+    if (!b) return fail(AMICA_E_INVALID, "null batch");
+    (void)hipSetDevice(b->device);
+    if (b->cap) (void)hipStreamSynchronize(b->cap);
+    b->drop_graph();
+    for (hipEvent_t ev : b->join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    for (amica_engine* e : b->eng)
+        if (e) amica_engine_destroy(e);
+    delete b;
+    return 0;
+}
+
+int amica_batch_create(const amica_options* opts, int nrec, amica_batch** out) {
+    if (!opts || !out || nrec < 1) return fail(AMICA_E_INVALID, "need options for nrec >= 1 recordings");
+    *out = nullptr;
+    for (int r = 1; r < nrec; ++r)
+        if (opts[r].device != opts[0].device) return fail(AMICA_E_INVALID, "all recordings of a batch share one device");
+    amica_batch* b = new amica_batch;
+    b->device = opts[0].device;
+    for (int r = 0; r < nrec; ++r) {
+        amica_engine* e = nullptr;
+        const int rc = amica_engine_create(&opts[r], &e);
+        if (rc) {
+            const std::string msg = "recording " + std::to_string(r) + ": " + g_last_error;
+            amica_batch_destroy(b);
+            return fail(rc, msg);
+        }
+        b->eng.push_back(e);
+    }
+    b->cus = b->eng[0]->cus;
+    b->join.assign(nrec, nullptr);
+    if (hipSetDevice(b->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+        amica_batch_destroy(b);
+        return fail(AMICA_E_DEVICE, "batch stream or event creation failed");
+    }
+    for (int r = 0; r < nrec; ++r)
+        if (hipEventCreateWithFlags(&b->join[r], hipEventDisableTiming) != hipSuccess) {
+            amica_batch_destroy(b);
+            return fail(AMICA_E_DEVICE, "batch event creation failed");
+        }
+    *out = b;
+    return 0;
+}
+
+int amica_batch_size(amica_batch* b) { return b ? (int)b->eng.size() : 0; }
+
+int amica_batch_engine(amica_batch* b, int r, amica_engine** eng) {
+    if (!b || !eng || r < 0 || r >= (int)b->eng.size()) return fail(AMICA_E_INVALID, "no such recording");
+    *eng = b->eng[r];
+    return 0;
+}
+
+int amica_batch_set_budgets(amica_batch* b, const int* cu_budget) {
+    if (!b) return fail(AMICA_E_INVALID, "null batch");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->cap));
+    for (size_t r = 0; r < b->eng.size(); ++r) {
+        HIPCHK(hipStreamSynchronize(b->eng[r]->st));
+        b->eng[r]->size_grids(cu_budget ? cu_budget[r] : b->cus);
+    }
+    b->drop_graph();
+    return 0;
+}
+
+int amica_batch_set_policy(amica_batch* b, int p) {
+    if (!b) return fail(AMICA_E_INVALID, "null batch");
+    if (p < 0 || p >= AMICA_BATCH_POLICIES) return fail(AMICA_E_INVALID, "no such policy");
+    const int R = (int)b->eng.size(), cus = b->cus;
+    double total = 0.0;
+    std::vector<double> work(R);
+    for (int r = 0; r < R; ++r) {
+        const amica_options& o = b->eng[r]->o;
+        work[r] = (double)o.M * o.m * o.n * (double)o.T;
+        total += work[r];
+    }
+    std::vector<int> bud(R);
+    for (int r = 0; r < R; ++r) {
+        const int share = std::max(std::max(1, cus / 16), (int)std::lround(cus * work[r] / total));
+        switch (p) {
+            case 0: bud[r] = cus; break;
+            case 1: bud[r] = std::min(cus, share); break;
+            case 2: bud[r] = std::min(cus, 2 * share); break;
+            default: bud[r] = std::max(1, cus / 2); break;
+        }
+    }
+    return amica_batch_set_budgets(b, bud.data());
+}
+
+int amica_batch_iterate(amica_batch* b, int k) {
+    if (!b) return fail(AMICA_E_INVALID, "null batch");
+    for (amica_engine* e : b->eng) {
+        const int rc = check_ready(e);
+        if (rc) return rc;
+    }
+    HIPCHK(hipSetDevice(b->device));
+    if (!b->gexec) {
+        const int rc = b->capture();
+        if (rc) return rc;
+    }
+    for (int it = 0; it < k; ++it) HIPCHK(hipGraphLaunch(b->gexec, b->cap));
+    /* work later enqueued on a recording's own stream (reads of its logs and
+     * state) orders after the batch */
+    HIPCHK(hipEventRecord(b->done, b->cap));
+    for (amica_engine* e : b->eng) HIPCHK(hipStreamWaitEvent(e->st, b->done, 0));
+    return 0;
+}
+
+int amica_batch_sync(amica_batch* b) {
+    if (!b) return fail(AMICA_E_INVALID, "null batch");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->cap));
+    return 0;
+}
+
+int amica_batch_tune(amica_batch* b, int iters, int* policy, double* ms) {
+    if (!b || iters < 1) return fail(AMICA_E_INVALID, "null batch or iters < 1");
+    for (amica_engine* e : b->eng) {
+        const int rc = check_ready(e);
+        if (rc) return rc;
+        if (e->o.max_iter < iters + 1) return fail(AMICA_E_INVALID, "max_iter of every recording must cover the tuning iterations");
+    }
+    HIPCHK(hipSetDevice(b->device));
+    /* the state every recording had on entry */
+    struct Snap { std::vector<double> A, c, alpha, mu, beta, rho, gm, W; };
+    std::vector<Snap> snap(b->eng.size());
+    for (size_t r = 0; r < b->eng.size(); ++r) {
+        const amica_options& o = b->eng[r]->o;
+        const size_t nn = (size_t)o.n * o.n * o.M, nm = (size_t)o.n * o.m * o.M;
+        Snap& s = snap[r];
+        s.A.resize(nn); s.W.resize(nn); s.c.resize((size_t)o.n * o.M);
+        s.alpha.resize(nm); s.mu.resize(nm); s.beta.resize(nm); s.rho.resize(nm); s.gm.resize(o.M);
+        const int rc = amica_engine_get_state(b->eng[r], s.A.data(), s.c.data(), s.alpha.data(),
+                                              s.mu.data(), s.beta.data(), s.rho.data(), s.gm.data(),
+                                              s.W.data());
+        if (rc) return rc;
+    }
+    auto restore = [&]() -> int {
+        for (size_t r = 0; r < b->eng.size(); ++r) {
+            const Snap& s = snap[r];
+            const int rc = amica_engine_set_state(b->eng[r], s.A.data(), s.c.data(), s.alpha.data(),
+                                                  s.mu.data(), s.beta.data(), s.rho.data(), s.gm.data());
+            if (rc) return rc;
+        }
+        return 0;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = HUGE_VAL;
+    int bestp = 0, rc = 0;
+    for (int p = 0; p < AMICA_BATCH_POLICIES && rc == 0; ++p) {
+        rc = amica_batch_set_policy(b, p);
+        if (!rc) rc = amica_batch_iterate(b, 1);            /* capture + untimed replay */
+        if (!rc && hipEventRecord(e0, b->cap) != hipSuccess) rc = AMICA_E_DEVICE;
+        if (!rc) rc = amica_batch_iterate(b, iters);
+        if (!rc && hipEventRecord(e1, b->cap) != hipSuccess) rc = AMICA_E_DEVICE;
+        if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = AMICA_E_DEVICE;
+        float el = 0.f;
+        if (!rc && hipEventElapsedTime(&el, e0, e1) != hipSuccess) rc = AMICA_E_DEVICE;
+        if (!rc) {
+            const double t = el / iters;
+            if (ms) ms[p] = t;
+            if (t < best) { best = t; bestp = p; }
+            rc = restore();
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc == AMICA_E_DEVICE ? fail(AMICA_E_DEVICE, "batch tuning failed") : rc;
+    rc = amica_batch_set_policy(b, bestp);
+    if (rc) return rc;
+    if (policy) *policy = bestp;
+    return restore();
 }
 
 }  // extern "C"

@@ -65,8 +65,9 @@ int launch_estep(const EstepArgs& a, int m, int nblocks, hipStream_t st);
 int estep_configure(int m, size_t lds_bytes);
 /* Dynamic LDS of one E-step workgroup of `threads` threads. */
 size_t estep_lds_bytes(int m, int threads);
-/* Resident E-step workgroups per CU (registers and LDS), or -1. */
-int estep_blocks_per_cu(int m, bool b64, int threads);
+/* Resident E-step workgroups per CU (registers and LDS), or -1; m1 for the
+ * single-model kernel (M = 1, no per-sample L recorded). */
+int estep_blocks_per_cu(int m, bool b64, int threads, bool m1);
 
 /* GEMMs ------------------------------------------------------------------ */
 /* b[t][c] = sum_k x[t][k] W[c][k] - wc[c]    (c < ldb, k < NP) */

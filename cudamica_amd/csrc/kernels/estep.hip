@@ -354,6 +354,179 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     }
 }
 
+/* M = 1: the model posterior v is 1, so the waves of a workgroup (the
+ * 64-channel blocks of one range of samples) never exchange anything: no
+ * barrier, no LDS, and phase 2 of a sample follows its phase 1 in
+ * registers.  One sample per iteration, loads 4 samples ahead; the
+ * statistics are summed in FP32 registers over 32 samples, in a second
+ * FP32 register set over 1024 samples and in FP64 in the workgroup's slot
+ * of the partial buffer.  The log-likelihood sum is sum_t sum_i q_ti (the
+ * model constant is added by the reduction); vsum is the sample count.
+ * Not used when per-sample L is recorded (ltall). */
+template <int MJ, bool B64>
+__global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
+    constexpr int NSJ = kStatsPerJ;
+    constexpr int NS = NSJ * MJ + kStatsPerI;
+    constexpr int G = 8;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = w * 64 + lane;
+    const bool act = i < a.n;
+    __shared__ double llw[kMaxEstepWaves];
+
+    float sb[MJ], nsbmu[MJ], rm2[MJ], cq2[MJ], rsb[MJ];
+    double sb64[B64 ? MJ : 1], nsbmu64[B64 ? MJ : 1];
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) {
+        const float* cj = a.cst + (size_t)j * a.NP + i;
+        const size_t ks = (size_t)MJ * a.NP;
+        sb[j] = act ? cj[kCstSb * ks] : 0.f;
+        nsbmu[j] = act ? cj[kCstNsbmu * ks] : 1.f;
+        rm2[j] = act ? cj[kCstRm2 * ks] : 0.f;
+        cq2[j] = act ? cj[kCstCq2 * ks] : 0.f;
+        rsb[j] = act ? cj[kCstRsb * ks] : 0.f;
+        if constexpr (B64) {
+            const double* dj = a.cst64 + (size_t)j * a.NP + i;
+            sb64[j] = act ? dj[kCst64Sb * ks] : 0.0;
+            nsbmu64[j] = act ? dj[kCst64Nsbmu * ks] : 1.0;
+        }
+    }
+    float acc[NS], acc2[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] = acc2[s] = 0.f;
+    float qacc = 0.f;
+    double qtot = 0.0;
+    double* const base = a.part + (size_t)blockIdx.x * a.pstride;
+    double* const l3 = base + (size_t)i * NS;
+    bool l3_started = false;
+
+    const long t0 = (long)blockIdx.x * a.chunk;
+    const long t1 = min(a.T, t0 + a.chunk);
+    const size_t bcol = (size_t)i;
+    uint4* const g8 = (uint4*)a.g;
+
+    /* a 4-sample prefetch window, rotated one sample per iteration */
+    float bq0, bq1, bq2, bq3, xq0, xq1, xq2, xq3;
+    double dq0 = 0.0, dq1 = 0.0, dq2 = 0.0, dq3 = 0.0;
+    auto ld = [&](long t, float& bq, double& dq, float& xq) {
+        const bool ok = act && t < t1;
+        if constexpr (B64) dq = ok ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
+        else bq = ok ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
+        xq = ok ? a.x[(size_t)t * a.NP + i] : 0.f;
+    };
+    ld(t0, bq0, dq0, xq0);
+    ld(t0 + 1, bq1, dq1, xq1);
+    ld(t0 + 2, bq2, dq2, xq2);
+    ld(t0 + 3, bq3, dq3, xq3);
+    /* packed bf16 g: pairs of samples shift into gw0..gw3 */
+    uint32_t gw0 = 0, gw1 = 0, gw2 = 0, gw3 = 0;
+    float gprev = 0.f;
+    const long tend = t0 + (t1 - t0 + G - 1) / G * G;   /* whole 8-sample groups */
+#pragma unroll 1
+    for (long t = t0; t < tend; ++t) {
+        const bool valid = t < t1;
+        const float bcur = B64 ? 0.f : bq0;
+        const double dcur = dq0;
+        const float xcur = xq0;
+        bq0 = bq1; bq1 = bq2; bq2 = bq3;
+        dq0 = dq1; dq1 = dq2; dq2 = dq3;
+        xq0 = xq1; xq1 = xq2; xq2 = xq3;
+        ld(t + 4, bq3, dq3, xq3);
+        const float bv = B64 ? (float)dcur : bcur;
+        float q2[MJ], E[MJ], SY[MJ], YM2[MJ], YR[MJ], LA[MJ];
+#pragma unroll
+        for (int j = 0; j < MJ; ++j) {
+            float y;
+            if constexpr (B64) y = (float)fma(sb64[j], dcur, nsbmu64[j]);
+            else y = fmaf(sb[j], bv, nsbmu[j]);
+            LA[j] = fast_log2(fmaxf(fabsf(y), 1.17549435e-38f));
+            YM2[j] = fast_exp2(rm2[j] * LA[j]);
+            SY[j] = YM2[j] * y;
+            YR[j] = SY[j] * y;
+            q2[j] = fmaf(YR[j], -AMICA_L2E, cq2[j]);
+        }
+        float se = 0.f, off = 0.f;
+#pragma unroll
+        for (int j = 0; j < MJ; ++j) {
+            E[j] = fast_exp2(q2[j]);
+            se += E[j];
+        }
+        if (__builtin_expect(__any(se < 0x1p-100f), 0)) {
+            off = q2[0];
+#pragma unroll
+            for (int j = 1; j < MJ; ++j) off = fmaxf(off, q2[j]);
+            se = 0.f;
+#pragma unroll
+            for (int j = 0; j < MJ; ++j) {
+                E[j] = fast_exp2(q2[j] - off);
+                se += E[j];
+            }
+        }
+        const bool use = act && valid;
+        const float rs = use ? fast_rcp(se) : 0.f;
+        qacc += use ? off + fast_log2(se) : 0.f;
+        float g = 0.f;
+#pragma unroll
+        for (int j = 0; j < MJ; ++j) {
+            const float u = E[j] * rs;
+            const float uf = u * SY[j];
+            g = fmaf(uf, rsb[j], g);
+            float* aj = acc + j * NSJ;
+            aj[0] += u;
+            aj[1] += uf;
+            aj[2] = fmaf(uf, SY[j], aj[2]);
+            aj[3] = fmaf(u, YM2[j], aj[3]);
+            const float uyr = u * YR[j];
+            aj[4] += uyr;
+            aj[5] = fmaf(uyr, YR[j], aj[5]);
+            aj[6] = fmaf(uyr, LA[j], aj[6]);
+        }
+        acc[NSJ * MJ + 0] = fmaf(bv, bv, acc[NSJ * MJ + 0]);
+        acc[NSJ * MJ + 1] += xcur;
+        acc[NSJ * MJ + 2] += g;
+        const int k = (int)(t - t0);                  /* t0 is a multiple of 8 */
+        if (a.g_bf16) {
+            if (k & 1) {
+                gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = pack_bf16x2(gprev, g);
+                if ((k & 7) == 7) g8[(size_t)((t - 7) >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
+            }
+            gprev = g;
+        } else if (valid) {
+            ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
+        }
+        if ((k & 31) == 31) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) { acc2[s] += acc[s]; acc[s] = 0.f; }
+            qtot += (double)qacc;
+            qacc = 0.f;
+            if ((k & 1023) == 1023) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    l3[s] = l3_started ? l3[s] + (double)acc2[s] : (double)acc2[s];
+                    acc2[s] = 0.f;
+                }
+                l3_started = true;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const double v = (double)acc2[s] + (double)acc[s];
+        const double r = l3_started ? l3[s] + v : v;
+        l3[s] = act ? r : 0.0;
+    }
+    qtot += (double)qacc;
+    const double ls = wave_sum_f64(qtot);
+    if (lane == 0) llw[w] = ls;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double l = 0.0;
+        for (int c = 0; c < (int)(blockDim.x >> 6); ++c) l += llw[c];
+        const size_t tail = (size_t)NS * a.NP;
+        base[tail] = (double)(t1 - t0);
+        base[tail + 1] = l * AMICA_LN2;
+    }
+}
+
 }  // namespace
 
 int estep_configure(int MJ, size_t lds_bytes) {
@@ -385,16 +558,22 @@ size_t estep_lds_bytes(int MJ, int threads) {
     return (size_t)((kStatsPerJ * MJ + kStatsPerI) | 1) * threads * sizeof(float);
 }
 
-int estep_blocks_per_cu(int MJ, bool b64, int threads) {
+int estep_blocks_per_cu(int MJ, bool b64, int threads, bool m1) {
     int nb = 0;
-    const size_t shm = estep_lds_bytes(MJ, threads);
+    const size_t shm = m1 ? 0 : estep_lds_bytes(MJ, threads);
     hipError_t r = hipErrorInvalidValue;
 #define AMICA_ESTEP_OCC(J)                                                                        \
     case J:                                                                                       \
-        r = b64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_kernel<J, true>,        \
-                                                               threads, shm)                      \
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_kernel<J, false>,       \
-                                                               threads, shm);                     \
+        if (m1)                                                                                   \
+            r = b64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_m1_kernel<J, true>, \
+                                                                   threads, 0)                    \
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_m1_kernel<J, false>,\
+                                                                   threads, 0);                   \
+        else                                                                                      \
+            r = b64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_kernel<J, true>,    \
+                                                                   threads, shm)                  \
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_kernel<J, false>,   \
+                                                                   threads, shm);                 \
         break;
     switch (MJ) {
         AMICA_ESTEP_OCC(1)
@@ -416,10 +595,16 @@ int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
     dim3 grid(nblocks), block(64 * a.M * a.CW);
     const size_t shm = estep_lds_bytes(MJ, (int)block.x);
     const bool b64 = a.b64 != nullptr;
+    const bool m1 = a.M == 1 && a.ltall == nullptr;
 #define AMICA_ESTEP_CASE(J)                                                                   \
     case J:                                                                                   \
-        if (b64) hipLaunchKernelGGL((estep_kernel<J, true>), grid, block, shm, st, a);        \
-        else hipLaunchKernelGGL((estep_kernel<J, false>), grid, block, shm, st, a);           \
+        if (m1) {                                                                             \
+            if (b64) hipLaunchKernelGGL((estep_m1_kernel<J, true>), grid, block, 0, st, a);   \
+            else hipLaunchKernelGGL((estep_m1_kernel<J, false>), grid, block, 0, st, a);      \
+        } else {                                                                              \
+            if (b64) hipLaunchKernelGGL((estep_kernel<J, true>), grid, block, shm, st, a);    \
+            else hipLaunchKernelGGL((estep_kernel<J, false>), grid, block, shm, st, a);       \
+        }                                                                                     \
         break;
     switch (MJ) {
         AMICA_ESTEP_CASE(1)

@@ -78,6 +78,15 @@ _SIGNATURES = [
     ("amica_synth_init", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
                                         ctypes.c_int, ctypes.c_double, _D, _D, _D, _D, _D, _D, _D]),
     ("amica_synth_cond", ctypes.c_double, [ctypes.c_int, _D]),
+    ("amica_batch_create", ctypes.c_int, [ctypes.POINTER(Options), ctypes.c_int, ctypes.POINTER(_V)]),
+    ("amica_batch_destroy", ctypes.c_int, [_V]),
+    ("amica_batch_size", ctypes.c_int, [_V]),
+    ("amica_batch_engine", ctypes.c_int, [_V, ctypes.c_int, ctypes.POINTER(_V)]),
+    ("amica_batch_set_budgets", ctypes.c_int, [_V, _I]),
+    ("amica_batch_set_policy", ctypes.c_int, [_V, ctypes.c_int]),
+    ("amica_batch_tune", ctypes.c_int, [_V, ctypes.c_int, _I, _D]),
+    ("amica_batch_iterate", ctypes.c_int, [_V, ctypes.c_int]),
+    ("amica_batch_sync", ctypes.c_int, [_V]),
     ("amica_ll_forward", ctypes.c_int, [ctypes.c_int, _V, ctypes.c_long, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int] + [_V] * 7 + [ctypes.c_double, _V, _V]),
     ("amica_ll_adjoint", ctypes.c_int, [ctypes.c_int, _V, ctypes.c_long, ctypes.c_int, ctypes.c_int,
@@ -142,16 +151,21 @@ def options(n, T, M=1, m=3, precision=PREC_FP32, max_iter=200, **kw):
 class Engine:
     """One recording on one GPU.  Arrays use the flat layouts of amica_engine.h."""
 
-    def __init__(self, opts):
+    def __init__(self, opts, handle=None):
+        """Creates an engine, or (handle given) wraps one a Batch owns."""
         self.L = load_library()
         self.o = opts
-        h = _V()
-        _check(self.L.amica_engine_create(ctypes.byref(opts), ctypes.byref(h)), "amica_engine_create")
-        self.h = h
+        self.owned = handle is None
+        if handle is None:
+            handle = _V()
+            _check(self.L.amica_engine_create(ctypes.byref(opts), ctypes.byref(handle)),
+                   "amica_engine_create")
+        self.h = handle
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.amica_engine_destroy(self.h)
+            if self.owned:
+                self.L.amica_engine_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -314,6 +328,65 @@ def synth_init(n, M, m, seed, fix_init=1, rho0=1.5):
 def cond(A):
     A = np.ascontiguousarray(np.asarray(A, np.float64).T)  # column-major flat
     return load_library().amica_synth_cond(A.shape[0], _d(A.ravel()))
+
+
+BATCH_POLICIES = ("whole GPU each", "work-proportional CUs", "twice proportional", "half GPU each")
+
+
+class Batch:
+    """Many recordings per EM iteration (include/amica_batch.h).  engines[r]
+    is recording r's engine (load data and a state through it)."""
+
+    def __init__(self, opts_list):
+        self.L = load_library()
+        arr = (Options * len(opts_list))(*opts_list)
+        h = _V()
+        _check(self.L.amica_batch_create(arr, len(opts_list), ctypes.byref(h)), "amica_batch_create")
+        self.h = h
+        self.engines = []
+        for r, o in enumerate(opts_list):
+            eh = _V()
+            _check(self.L.amica_batch_engine(self.h, r, ctypes.byref(eh)), "amica_batch_engine")
+            self.engines.append(Engine(o, handle=eh))
+
+    def close(self):
+        if getattr(self, "h", None):
+            for e in self.engines:
+                e.close()
+            self.L.amica_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_budgets(self, budgets=None):
+        arr = None if budgets is None else _i(np.ascontiguousarray(budgets, dtype=np.int32))
+        _check(self.L.amica_batch_set_budgets(self.h, arr), "amica_batch_set_budgets")
+
+    def set_policy(self, p):
+        _check(self.L.amica_batch_set_policy(self.h, p), "amica_batch_set_policy")
+
+    def tune(self, iters=3):
+        """Returns (chosen policy, ms per batched iteration for every policy)."""
+        p = ctypes.c_int(0)
+        ms = np.zeros(len(BATCH_POLICIES))
+        _check(self.L.amica_batch_tune(self.h, iters, ctypes.byref(p), _d(ms)), "amica_batch_tune")
+        return p.value, ms
+
+    def iterate(self, k=1):
+        _check(self.L.amica_batch_iterate(self.h, k), "amica_batch_iterate")
+
+    def sync(self):
+        _check(self.L.amica_batch_sync(self.h), "amica_batch_sync")
 
 
 # per-sample log-likelihood and its adjoint (include/amica_adjoint.h) -------

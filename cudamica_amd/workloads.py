@@ -6,7 +6,11 @@ include/amica_synth.h).  The initial state uses the seed + 0x1000 stream.
 """
 from dataclasses import dataclass, field
 
+import numpy as np
+
 from . import engine as _e
+
+_U64 = (1 << 64) - 1
 
 
 @dataclass
@@ -36,7 +40,8 @@ class Workload:
                              sub_frac=self.sub_frac, cond_max=self.cond_max, noise_std=self.noise_std)
 
     def init_state(self):
-        return _e.synth_init(self.n, self.M, self.m, self.seed + 0x1000, fix_init=1, rho0=1.5)
+        return _e.synth_init(self.n, self.M, self.m, (self.seed + 0x1000) & _U64, fix_init=1,
+                             rho0=1.5)
 
     def options(self, T=None, precision=None, max_iter=None, **kw):
         return _e.options(self.n, self.T if T is None else T, self.M, self.m,
@@ -61,3 +66,20 @@ WORKLOADS = {
                             seed=0x5EED0004, nmix=2, block=100_000, alternate=1, cond_max=0.0,
                             precision=_e.PREC_BF16),
 }
+
+
+def batch_workload(count=32, seed=0x5EED0003, precision=_e.PREC_FP32, t_range=(100_000, 1_500_000),
+                   n_choices=(32, 64, 128), iters=200):
+    """Many independent recordings per EM iteration: n drawn uniformly from
+    n_choices, T uniformly from t_range, M = 1, m = 3, rho ~ U[1, 2], mixing
+    condition number <= 100.  Recording r uses the data seed
+    seed ^ ((r + 1) * 0x9E3779B97F4A7C15) (64-bit)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(count):
+        n = int(rng.choice(n_choices))
+        T = int(rng.integers(t_range[0], t_range[1] + 1))
+        s = (seed ^ ((r + 1) * 0x9E3779B97F4A7C15)) & _U64
+        out.append(Workload(f"rec{r}_n{n}_T{T}", n=n, T=T, M=1, m=3, iters=iters, seed=s,
+                            rho=(1.0, 2.0), cond_max=100.0, precision=precision))
+    return out
