@@ -113,12 +113,56 @@ def measured_traffic(workload, prec):
         return json.load(f).get("traffic_bytes")
 
 
+def bench_batch(args):
+    """Many recordings per EM iteration (one hipGraph): the batch workload of
+    32 recordings, n in {32, 64, 128}, T ~ U[100k, 1.5M], M = 1, m = 3."""
+    import cudamica_amd as ca
+    prec = 16 if args.precision == "bf16" else 32
+    ws = ca.batch_workload(precision=prec)
+    t0 = time.perf_counter()
+    opts = [w.options(max_iter=args.warmup + args.steps + 4) for w in ws]
+    b = ca.Batch(opts)
+    for w, e in zip(ws, b.engines):
+        e.load_raw(ca.synth_data(w.spec()))
+        e.preprocess(w.do_sphere)
+        e.set_state(w.init_state())
+    log(f"batch of {len(ws)} recordings ({sum(w.n * w.T for w in ws) / 1e9:.2f} G channel-samples) "
+        f"ready in {time.perf_counter() - t0:.1f} s")
+    p, ms = b.tune(iters=2)
+    log("tuner (ms per batched iteration): " +
+        ", ".join(f"{nm} {t:.2f}" for nm, t in zip(ca.BATCH_POLICIES, ms)) +
+        f"; using {ca.BATCH_POLICIES[p]}")
+    b.iterate(args.warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.iterate(args.steps)
+    b.sync()
+    dt = time.perf_counter() - t0
+    lls = [float(e.log(args.warmup + args.steps)["ll"][-1]) for e in b.engines]
+    b.close()
+    out = {
+        "metric": "batched EM iters/s (32 recordings per iteration, one hipGraph)",
+        "value": args.steps / dt, "unit": "batched EM iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16+f32" if prec == 16 else "f32", "data": "synthetic",
+        "config": {"workload": "batch32_n32-128_T100k-1.5M_M1", "recordings": len(ws),
+                   "precision": "bf16" if prec == 16 else "fp32", "parallelism": "single-gpu"},
+        "tuner_ms": dict(zip(ca.BATCH_POLICIES, [float(t) for t in ms])),
+        "policy": ca.BATCH_POLICIES[p],
+        "recording_iters_per_s": len(ws) * args.steps / dt,
+        "ll_finite": bool(np.all(np.isfinite(lls))),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="n128_T2M_M4")
+    ap.add_argument("--workload", default="n128_T2M_M4",
+                    help="a workload of cudamica_amd.WORKLOADS, or batch32 (32 recordings)")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default=None)
     ap.add_argument("--T", type=int, default=None, help="override the sample count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -130,6 +174,9 @@ def main():
               "(one GPU owns all samples); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch32":
+        bench_batch(args)
+        return
     import cudamica_amd as ca
     w = ca.WORKLOADS[args.workload]
     if args.T:
