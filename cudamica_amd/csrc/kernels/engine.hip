@@ -777,6 +777,7 @@ struct amica_batch {
     std::vector<hipEvent_t> join;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    int mode = 0;                       /* launch mode of the current policy */
 
     void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -887,6 +888,8 @@ int amica_batch_set_budgets(amica_batch* b, const int* cu_budget) {
 int amica_batch_set_policy(amica_batch* b, int p) {
     if (!b) return fail(AMICA_E_INVALID, "null batch");
     if (p < 0 || p >= AMICA_BATCH_POLICIES) return fail(AMICA_E_INVALID, "no such policy");
+    b->mode = p / 4;
+    p %= 4;
     const int R = (int)b->eng.size(), cus = b->cus;
     double total = 0.0;
     std::vector<double> work(R);
@@ -915,6 +918,19 @@ int amica_batch_iterate(amica_batch* b, int k) {
         if (rc) return rc;
     }
     HIPCHK(hipSetDevice(b->device));
+    if (b->mode == 1) {
+        /* each recording replays its own graph on its own stream */
+        HIPCHK(hipEventRecord(b->fork, b->cap));
+        for (size_t r = 0; r < b->eng.size(); ++r) {
+            amica_engine* e = b->eng[r];
+            HIPCHK(hipStreamWaitEvent(e->st, b->fork, 0));
+            const int rc = amica_engine_iterate_graph(e, k);
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(b->join[r], e->st));
+            HIPCHK(hipStreamWaitEvent(b->cap, b->join[r], 0));
+        }
+        return 0;
+    }
     if (!b->gexec) {
         const int rc = b->capture();
         if (rc) return rc;

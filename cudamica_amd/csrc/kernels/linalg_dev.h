@@ -104,70 +104,6 @@ __device__ inline double block_lu_inverse(const double* __restrict__ A, int n,
     return lad_sh;
 }
 
-/* In-place Gauss-Jordan inverse of the n x n column-major matrix in LDS
- * `a` with partial pivoting (ties to the smallest row index), by one
- * workgroup.  prow and pcol are n-double LDS rows, swp an n-int LDS array.
- * The row interchanges are undone as column interchanges in reverse order
- * at the end, so `a` holds A^-1.  Returns sum_k log|pivot_k| = log|det A|
- * to every thread. */
-__device__ inline double block_gj_inverse_lds(double* a, int n, double* prow, double* pcol,
-                                              int* swp) {
-    __shared__ int piv_sh;
-    __shared__ double lad_sh;
-    const int tid = threadIdx.x, lane = tid & 63, bd = blockDim.x;
-    const long nn = (long)n * n;
-    if (tid == 0) lad_sh = 0.0;
-    for (int k = 0; k < n; ++k) {
-        if (tid < 64) {
-            double best = -1.0;
-            int bi = n;
-            for (int i = k + lane; i < n; i += 64) {
-                const double v = fabs(a[i + (long)k * n]);
-                if (v > best) { best = v; bi = i; }
-            }
-            for (int off = 32; off >= 1; off >>= 1) {
-                const double ob = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-            }
-            if (lane == 0) { piv_sh = bi; swp[k] = bi; }
-        }
-        __syncthreads();
-        const int p = piv_sh;
-        if (p != k) {
-            for (int j = tid; j < n; j += bd) {
-                const double t1 = a[k + (long)j * n];
-                a[k + (long)j * n] = a[p + (long)j * n];
-                a[p + (long)j * n] = t1;
-            }
-            __syncthreads();
-        }
-        const double d = a[k + (long)k * n];
-        const double inv = 1.0 / d;
-        for (int j = tid; j < n; j += bd) prow[j] = (j == k ? 1.0 : a[k + (long)j * n]) * inv;
-        for (int i = tid; i < n; i += bd) pcol[i] = a[i + (long)k * n];
-        if (tid == 0) lad_sh += log(fabs(d));
-        __syncthreads();
-        for (long q = tid; q < nn; q += bd) {
-            const int i = (int)(q % n), j = (int)(q / n);
-            a[q] = (i == k) ? prow[j] : ((j == k) ? 0.0 : a[q]) - pcol[i] * prow[j];
-        }
-        __syncthreads();
-    }
-    for (int k = n - 1; k >= 0; --k) {
-        const int p = swp[k];
-        if (p != k) {
-            for (int i = tid; i < n; i += bd) {
-                const double t1 = a[i + (long)k * n];
-                a[i + (long)k * n] = a[i + (long)p * n];
-                a[i + (long)p * n] = t1;
-            }
-            __syncthreads();
-        }
-    }
-    return lad_sh;
-}
-
 }  // namespace amica
 
 #endif

@@ -21,6 +21,8 @@
  * and log|det W| = -sum log|pivot| (the reference multiplies the diagonal
  * of the LU of W, which can overflow).
  */
+#include <algorithm>
+
 #include "common.h"
 #include "engine_internal.h"
 #include "linalg_dev.h"
@@ -58,13 +60,27 @@ __device__ double block_sum(double v, double* sh) {
 /* reductions                                                                 */
 /* ------------------------------------------------------------------------ */
 
-__global__ void sum_parts_kernel(const double* __restrict__ part, int nparts, size_t pstride,
-                                 size_t len, double* __restrict__ out) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= len) return;
+/* out[e] = sum_p part[p][e].  A workgroup covers 64 elements (lanes); its
+ * kSumWaves waves take the partials p = w, w + kSumWaves, ... and their
+ * totals are added in wave order, so the result does not depend on timing. */
+constexpr int kSumWaves = 16;
+
+__global__ __launch_bounds__(64 * kSumWaves) void sum_parts_kernel(const double* __restrict__ part,
+                                                                    int nparts, size_t pstride,
+                                                                    size_t len, double* __restrict__ out) {
+    __shared__ double acc[kSumWaves][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const size_t e = (size_t)blockIdx.x * 64 + lane;
     double s = 0.0;
-    for (int p = 0; p < nparts; ++p) s += part[(size_t)p * pstride + e];
-    out[e] = s;
+    if (e < len)
+        for (int p = w; p < nparts; p += kSumWaves) s += part[(size_t)p * pstride + e];
+    acc[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && e < len) {
+        double t = 0.0;
+        for (int q = 0; q < kSumWaves; ++q) t += acc[q][lane];
+        out[e] = t;
+    }
 }
 
 /* Canonical statistics of model blockIdx.x (layout of the oracle, see the
@@ -347,94 +363,174 @@ __device__ __forceinline__ double joined_w(uint16_t hi, uint16_t lo) {
     return (double)bf16_bits_to_f32(hi) + (double)bf16_bits_to_f32(lo);
 }
 
-/* Everything after W = A^-1: the GEMM copies of W (rows h NP + i, columns
- * k), W~ c with the W~ the unmixing GEMM multiplies by, the E-step
- * constants and log|det W|.  Wl is W (column-major) in LDS or global. */
-__device__ void prepare_tail(const MstepArgs& a, int h, const double* Wl, double lad) {
-    const int n = a.n, m = a.m, NP = a.NP, tid = threadIdx.x;
+/* Everything after W = A^-1, for all models at once (grid (., M)): the GEMM
+ * copies of W (rows h NP + i, columns k), W~ c with the W~ the unmixing GEMM
+ * multiplies by, and the per-(h, j, i) E-step constants.  Flat index per
+ * model: [0, n n) W entries, [n n, n n + n) rows of W~ c, then the n m
+ * constants. */
+__global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
+    const int h = blockIdx.y, n = a.n, m = a.m, NP = a.NP;
     const long nn = (long)n * n, nm = (long)n * m;
-    double* W = a.W + (size_t)h * nn;
-    for (long q = tid; q < nn; q += blockDim.x) {
-        const int i = (int)(q % n), k = (int)(q / n);
-        const double wv2 = Wl[q];
-        W[q] = wv2;
-        const size_t dst = ((size_t)h * NP + i) * NP + k;
-        if (a.prec_bf16) {
-            uint16_t hi, lo;
-            split_w(wv2, hi, lo);
-            a.Whi[dst] = hi;
-            a.Wlo[dst] = lo;
-        } else {
-            a.Wg64[dst] = wv2;
-        }
-    }
-    const double* c = a.c + (size_t)h * n;
-    for (int i = tid; i < n; i += blockDim.x) {
-        double s = 0.0;
-        for (int k = 0; k < n; ++k) {
-            double wr = Wl[i + (long)k * n];
+    const double* W = a.W + (size_t)h * nn;
+    const long total = nn + n + nm;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        if (q < nn) {
+            const int i = (int)(q % n), k = (int)(q / n);
+            const double wv2 = W[q];
+            const size_t dst = ((size_t)h * NP + i) * NP + k;
             if (a.prec_bf16) {
                 uint16_t hi, lo;
-                split_w(wr, hi, lo);
-                wr = joined_w(hi, lo);
+                split_w(wv2, hi, lo);
+                a.Whi[dst] = hi;
+                a.Wlo[dst] = lo;
+            } else {
+                a.Wg64[dst] = wv2;
             }
-            s += wr * c[k];
+        } else if (q < nn + n) {
+            const int i = (int)(q - nn);
+            const double* c = a.c + (size_t)h * n;
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) {
+                double wr = W[i + (long)k * n];
+                if (a.prec_bf16) {
+                    uint16_t hi, lo;
+                    split_w(wr, hi, lo);
+                    wr = joined_w(hi, lo);
+                }
+                s += wr * c[k];
+            }
+            if (a.prec_bf16) a.wc[(size_t)h * NP + i] = (float)s;
+            else a.wc64[(size_t)h * NP + i] = s;
+        } else {
+            const long e = q - nn - n;
+            const int j = (int)(e / n), i = (int)(e % n);
+            const double al = a.alpha[(size_t)h * nm + e], mv = a.mu[(size_t)h * nm + e];
+            const double be = a.beta[(size_t)h * nm + e], r = a.rho[(size_t)h * nm + e];
+            const double sb = sqrt(be);
+            const double cq = log(al) + 0.5 * log(be) - AMICA_LOG2_REF - lgamma(1.0 + 1.0 / r);
+            float* cj = a.cst + ((size_t)h * kNumCst * m + j) * NP + i;
+            const size_t ks = (size_t)m * NP;
+            cj[kCstSb * ks] = (float)sb;
+            cj[kCstNsbmu * ks] = (float)(-sb * mv);
+            cj[kCstRm2 * ks] = (float)(r - 2.0);
+            cj[kCstCq2 * ks] = (float)(cq * 1.4426950408889634);
+            cj[kCstRsb * ks] = (float)(r * sb);
+            if (!a.prec_bf16) {
+                double* dj = a.cst64 + ((size_t)h * kNumCst64 * m + j) * NP + i;
+                dj[kCst64Sb * ks] = sb;
+                dj[kCst64Nsbmu * ks] = -sb * mv;
+            }
         }
-        if (a.prec_bf16) a.wc[(size_t)h * NP + i] = (float)s;
-        else a.wc64[(size_t)h * NP + i] = s;
-    }
-    /* E-step constants */
-    for (long q = tid; q < nm; q += blockDim.x) {
-        const int j = (int)(q / n), i = (int)(q % n);
-        const double al = a.alpha[(size_t)h * nm + q], mv = a.mu[(size_t)h * nm + q];
-        const double be = a.beta[(size_t)h * nm + q], r = a.rho[(size_t)h * nm + q];
-        const double sb = sqrt(be);
-        const double cq = log(al) + 0.5 * log(be) - AMICA_LOG2_REF - lgamma(1.0 + 1.0 / r);
-        float* cj = a.cst + ((size_t)h * kNumCst * m + j) * NP + i;
-        const size_t ks = (size_t)m * NP;
-        cj[kCstSb * ks] = (float)sb;
-        cj[kCstNsbmu * ks] = (float)(-sb * mv);
-        cj[kCstRm2 * ks] = (float)(r - 2.0);
-        cj[kCstCq2 * ks] = (float)(cq * 1.4426950408889634);
-        cj[kCstRsb * ks] = (float)(r * sb);
-        if (!a.prec_bf16) {
-            double* dj = a.cst64 + ((size_t)h * kNumCst64 * m + j) * NP + i;
-            dj[kCst64Sb * ks] = sb;
-            dj[kCst64Nsbmu * ks] = -sb * mv;
-        }
-    }
-    if (tid == 0) {
-        const double ldet = -lad;
-        a.ldet[h] = ldet;
-        a.lconst[h] = log(a.gm[h]) + ldet + a.ds->ldetS;
     }
 }
 
-/* n <= kLdsInvMax: W = A^-1 by Gauss-Jordan in LDS, one workgroup per model. */
+__device__ __forceinline__ void set_ldet(const MstepArgs& a, int h, double lad) {
+    const double ldet = -lad;          /* log|det W| = -log|det A| */
+    a.ldet[h] = ldet;
+    a.lconst[h] = log(a.gm[h]) + ldet + a.ds->ldetS;
+}
+
+/* n <= kLdsInvMax: W = A^-1 by Gauss-Jordan in LDS with implicit partial
+ * pivoting, one workgroup per model.  Rows are never moved: step k takes
+ * the pivot p_k with the largest |a_ik| among rows not pivoted yet (ties to
+ * the smallest index) and eliminates in place, so that afterwards
+ * a[p_r][j] = (A^-1)[r][p_j]; W is gathered from that.  The wave that
+ * updates the next pivot column takes its argmax while it writes it, which
+ * leaves two barriers per step. */
 constexpr int kLdsInvMax = 128;
 
-__global__ __launch_bounds__(512) void prepare_lds_kernel(MstepArgs a) {
-    extern __shared__ double sm[];
-    __shared__ int swp[kLdsInvMax];
-    const int h = blockIdx.x, n = a.n;
+__global__ __launch_bounds__(1024) void inverse_lds_kernel(MstepArgs a) {
+    extern __shared__ double sm[];               /* a[n n] (column-major), prow[n], pcol[n] */
+    __shared__ int piv[kLdsInvMax], pinv[kLdsInvMax], used[kLdsInvMax];
+    __shared__ int cand_i[16];
+    __shared__ double lad_sh;
+    const int h = blockIdx.x, n = a.n, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int bd = blockDim.x, nw = bd >> 6;
     const long nn = (long)n * n;
     double* aw = sm;
+    double* prow = sm + nn;
+    double* pcol = prow + n;
     const double* A = a.A + (size_t)h * nn;
-    for (long q = threadIdx.x; q < nn; q += blockDim.x) aw[q] = A[q];
+    for (long q = tid; q < nn; q += bd) aw[q] = A[q];
+    for (int i = tid; i < n; i += bd) used[i] = 0;
+    if (tid == 0) lad_sh = 0.0;
     __syncthreads();
-    const double lad = block_gj_inverse_lds(aw, n, sm + nn, sm + nn + n, swp);
-    prepare_tail(a, h, aw, lad);
+    /* lanes run over rows, waves over columns (no index division); the
+     * candidates of column c come from wave c % nw */
+    if (wv == 0) {
+        double cv = -1.0;
+        int ci = n;
+        for (int i = lane; i < n; i += 64) {
+            const double av = fabs(aw[i]);
+            if (av > cv) { cv = av; ci = i; }
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double ov = __shfl_xor(cv, off, 64);
+            const int oi = __shfl_xor(ci, off, 64);
+            if (ov > cv || (ov == cv && oi < ci)) { cv = ov; ci = oi; }
+        }
+        if (lane == 0) cand_i[0] = ci;
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const int p = cand_i[k % nw];
+        const double d = aw[p + (long)k * n];
+        const double inv = 1.0 / d;
+        if (tid < n) {
+            prow[tid] = (tid == k) ? inv : aw[p + (long)tid * n] * inv;
+        } else if (tid < 2 * n) {
+            const int i = tid - n;
+            pcol[i] = (i == p) ? 0.0 : aw[i + (long)k * n];
+        }
+        if (tid == 0) {
+            lad_sh += log(fabs(d));
+            piv[k] = p;
+            used[p] = 1;
+        }
+        __syncthreads();
+        for (int j = wv; j < n; j += nw) {
+            const double pj = prow[j];
+            double* col = aw + (long)j * n;
+            double cv = -1.0;
+            int ci = n;
+            for (int i = lane; i < n; i += 64) {
+                const double v = (i == p) ? pj : ((j == k) ? 0.0 : col[i]) - pcol[i] * pj;
+                col[i] = v;
+                if (j == k + 1 && !used[i]) {
+                    const double av = fabs(v);
+                    if (av > cv) { cv = av; ci = i; }
+                }
+            }
+            if (j == k + 1) {   /* wave-uniform: this wave holds the next pivot column */
+                for (int off = 32; off >= 1; off >>= 1) {
+                    const double ov = __shfl_xor(cv, off, 64);
+                    const int oi = __shfl_xor(ci, off, 64);
+                    if (ov > cv || (ov == cv && oi < ci)) { cv = ov; ci = oi; }
+                }
+                if (lane == 0) cand_i[j % nw] = ci;
+            }
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < n; r += bd) pinv[piv[r]] = r;
+    __syncthreads();
+    double* W = a.W + (size_t)h * nn;
+    for (int mcol = wv; mcol < n; mcol += nw) {
+        const double* src = aw + (long)pinv[mcol] * n;
+        for (int r = lane; r < n; r += 64) W[r + (long)mcol * n] = src[piv[r]];
+    }
+    if (tid == 0) set_ldet(a, h, lad_sh);
 }
 
 /* larger n: LU with partial pivoting in global memory */
-__global__ __launch_bounds__(kBlk) void prepare_kernel(MstepArgs a) {
+__global__ __launch_bounds__(kBlk) void inverse_lu_kernel(MstepArgs a) {
     __shared__ int perm[1024];
     const int h = blockIdx.x, n = a.n;
     const long nn = (long)n * n;
     double* W = a.W + (size_t)h * nn;
     double* X = a.work + 8 + (size_t)a.M * nn + (size_t)h * nn;
     const double lad = block_lu_inverse(a.A + (size_t)h * nn, n, a.lu + (size_t)h * nn, X, W, perm);
-    prepare_tail(a, h, W, lad);
+    if (threadIdx.x == 0) set_ldet(a, h, lad);
 }
 
 }  // namespace
@@ -443,13 +539,13 @@ int launch_reduce(const double* part, int nparts, size_t pstride, const double* 
                   int nchunks, double* raw, double* gxtot, const MstepArgs& ma, hipStream_t st) {
     const int NS = kStatsPerJ * ma.m + kStatsPerI;
     const size_t plen = (size_t)ma.M * NS * ma.NP + ma.M + 1;
-    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((plen + 255) / 256)), dim3(256), 0, st,
-                       part, nparts, pstride, plen, raw);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((plen + 63) / 64)), dim3(64 * kSumWaves), 0,
+                       st, part, nparts, pstride, plen, raw);
     const size_t glen = (size_t)ma.ldb * ma.NP;
-    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((glen + 255) / 256)), dim3(256), 0, st,
-                       gxpart, nchunks, glen, glen, gxtot);
-    const int ysplit = (ma.n + 31) / 32;
-    hipLaunchKernelGGL(finalize_kernel, dim3(ma.M, ysplit), dim3(kBlk), 0, st, raw, gxtot, ma);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((glen + 63) / 64)), dim3(64 * kSumWaves), 0,
+                       st, gxpart, nchunks, glen, glen, gxtot);
+    /* one phi row per workgroup (threads over its columns) */
+    hipLaunchKernelGGL(finalize_kernel, dim3(ma.M, ma.n), dim3(kBlk), 0, st, raw, gxtot, ma);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -462,21 +558,24 @@ int launch_mstep(const MstepArgs& ma, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-static size_t prepare_lds_bytes(int n) { return ((size_t)n * n + 2 * (size_t)n) * sizeof(double); }
+static size_t inverse_lds_bytes(int n) { return ((size_t)n * n + 2 * (size_t)n) * sizeof(double); }
 
 int mstep_configure(int n) {
     if (n > kLdsInvMax) return 0;
-    return hipFuncSetAttribute((const void*)prepare_lds_kernel,
+    return hipFuncSetAttribute((const void*)inverse_lds_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)prepare_lds_bytes(n)) == hipSuccess ? 0 : -1;
+                               (int)inverse_lds_bytes(n)) == hipSuccess ? 0 : -1;
 }
 
 int launch_prepare(const MstepArgs& ma, hipStream_t st) {
     if (ma.n > 1024) return -1;
     if (ma.n <= kLdsInvMax)
-        hipLaunchKernelGGL(prepare_lds_kernel, dim3(ma.M), dim3(512), prepare_lds_bytes(ma.n), st, ma);
+        hipLaunchKernelGGL(inverse_lds_kernel, dim3(ma.M), dim3(1024), inverse_lds_bytes(ma.n), st, ma);
     else
-        hipLaunchKernelGGL(prepare_kernel, dim3(ma.M), dim3(kBlk), 0, st, ma);
+        hipLaunchKernelGGL(inverse_lu_kernel, dim3(ma.M), dim3(kBlk), 0, st, ma);
+    const long total = (long)ma.n * ma.n + ma.n + (long)ma.n * ma.m;
+    const unsigned gx = (unsigned)std::min<long>((total + 255) / 256, 256);
+    hipLaunchKernelGGL(prepare_tail_kernel, dim3(gx, ma.M), dim3(256), 0, st, ma);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

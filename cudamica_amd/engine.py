@@ -330,7 +330,9 @@ def cond(A):
     return load_library().amica_synth_cond(A.shape[0], _d(A.ravel()))
 
 
-BATCH_POLICIES = ("whole GPU each", "work-proportional CUs", "twice proportional", "half GPU each")
+BATCH_POLICIES = tuple(f"{mode}, {bud}" for mode in ("one graph", "graph per stream")
+                       for bud in ("whole GPU each", "work-proportional CUs", "twice proportional",
+                                   "half GPU each"))
 
 
 class Batch:

@@ -27,10 +27,13 @@ extern "C" {
 
 typedef struct amica_batch amica_batch;
 
-/* CU-budget policies the tuner measures: every recording sized for the whole
- * GPU; budgets proportional to each recording's E-step work (M m n T);
- * twice that; half the GPU each. */
-#define AMICA_BATCH_POLICIES 4
+/* Policies the tuner measures: p = 4 * launch mode + CU budget.
+ * Launch modes: 0, the whole batch as one hipGraph (one branch per
+ * recording); 1, every recording replays its own one-iteration graph on its
+ * own stream (the streams join at the end of amica_batch_iterate).
+ * CU budgets: every recording sized for the whole GPU; budgets proportional
+ * to each recording's E-step work (M m n T); twice that; half the GPU each. */
+#define AMICA_BATCH_POLICIES 8
 
 /* nrec recordings; opts[r] as for amica_engine_create (same device). */
 int amica_batch_create(const amica_options* opts, int nrec, amica_batch** out);
@@ -48,7 +51,7 @@ int amica_batch_set_policy(amica_batch* b, int p);
  * Call after every recording has data and a state (set_state) and before
  * iterating: the tuner restores the state each recording had on entry. */
 int amica_batch_tune(amica_batch* b, int iters, int* policy, double* ms);
-/* k EM iterations of every recording, each one launch of the batch graph. */
+/* k EM iterations of every recording. */
 int amica_batch_iterate(amica_batch* b, int k);
 int amica_batch_sync(amica_batch* b);
 

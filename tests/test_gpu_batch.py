@@ -3,7 +3,8 @@ EM iteration over recordings of uneven n, T and M.
 
 * With every recording's grids sized for the whole GPU, the batch computes
   exactly what each recording's own engine computes (same partitions, fixed
-  reduction order): identical LL, lrate and parameters, bit for bit.
+  reduction order): identical LL, lrate and parameters, bit for bit, both
+  as one graph and as one graph per recording on its own stream.
 * The measured tuner times every CU-budget policy, applies the fastest and
   leaves each recording in the state it had; iterating afterwards gives the
   first LL of the solo engine to 1e-6 (other grids sum in another order).
@@ -39,10 +40,11 @@ def _load(eng, x, st):
     eng.set_state(st)
 
 
-@pytest.mark.parametrize("precision", [ca.PREC_FP32, ca.PREC_BF16])
-def test_batch_equals_solo_engines(built, precision):
+@pytest.mark.parametrize("precision,policy", [(ca.PREC_FP32, 0), (ca.PREC_BF16, 0), (ca.PREC_FP32, 4)])
+def test_batch_equals_solo_engines(built, precision, policy):
     iters = 6
-    _say(f"[batch] {len(RECS)} recordings, precision {precision}, {iters} iterations, whole-GPU grids")
+    _say(f"[batch] {len(RECS)} recordings, precision {precision}, {iters} iterations, "
+         f"{ca.BATCH_POLICIES[policy]}")
     opts, data, states = _setup(precision, iters)
     solo = []
     for o, x, st in zip(opts, data, states):
@@ -54,7 +56,7 @@ def test_batch_equals_solo_engines(built, precision):
     with ca.Batch(opts) as b:
         for e, x, st in zip(b.engines, data, states):
             _load(e, x, st)
-        b.set_policy(0)
+        b.set_policy(policy)
         b.iterate(iters)
         b.sync()
         for r, e in enumerate(b.engines):
