@@ -140,12 +140,7 @@ struct amica_engine {
         chunk = (T + nblocks - 1) / nblocks;
         chunk = ((chunk + 31) / 32) * 32;     /* whole packed 8-sample groups */
         nblocks = (int)((T + chunk - 1) / chunk);
-        if (bf16) {
-            nchunks = gx_bf16_chunks(T, NP, cu_budget);
-        } else {
-            const int tiles = (ldb / 64) * (NP / 64);
-            nchunks = (int)std::max(1L, std::min((long)(2 * cu_budget + tiles - 1) / tiles, (T + 31) / 32));
-        }
+        nchunks = bf16 ? gx_bf16_chunks(T, NP, cu_budget) : gx_f32_chunks(T, NP, ldb, cu_budget);
         if (gexec) {   /* the captured grids are stale */
             (void)hipGraphExecDestroy(gexec);
             (void)hipGraphDestroy(graph);

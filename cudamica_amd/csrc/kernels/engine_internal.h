@@ -86,8 +86,10 @@ int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, 
  * T rounded up to 32, zero padded); part is [nchunks][ldb][NP]. */
 int launch_gx_bf16(const uint16_t* g8, const uint16_t* x8, double* part, long T, int NP, int ldb,
                    int nchunks, hipStream_t st);
-/* Number of sample chunks (partials) launch_gx_bf16 should be given. */
+/* Number of sample chunks (partials) launch_gx_bf16 / launch_gx_f32 should
+ * be given for a budget of `cus` CUs (nondecreasing in cus). */
 int gx_bf16_chunks(long T, int NP, int cus);
+int gx_f32_chunks(long T, int NP, int ldb, int cus);
 /* x8[(t/8) NP + i][t%8] = xhi[t][i] (the packed B operand of the BF16
  * statistic GEMM); padding stays as allocated (zero). */
 int launch_pack8(const uint16_t* xhi, long T, int NP, uint16_t* x8, hipStream_t st);

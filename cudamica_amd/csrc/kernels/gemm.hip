@@ -243,55 +243,62 @@ __global__ __launch_bounds__(512) void unmix_bf16x3_reg_kernel(const uint16_t* _
 }
 
 /* ---------------- statistic GEMM G = g^T x, FP32 --------------------- */
-/* The reduction index is the sample, the row index of both operands, so
- * the 16x16x4 fragments are coalesced rows read straight from global
- * memory.  Workgroup tile 64 (g columns) x 64 (x columns) over a chunk of
- * samples; wave w owns x columns [16w, 16w+16) and all four 16-row tiles.
- * FP32 accumulation over 64 samples, then FP64. */
-__global__ __launch_bounds__(256) void gx_f32_kernel(const float* __restrict__ g,
-                                                     const float* __restrict__ x,
-                                                     double* __restrict__ part, long T, int NP,
-                                                     int ldb, long chunk) {
+/* FP32 statistic GEMM in slabs: workgroup = ldb/64 waves over one chunk of
+ * samples (at most a few thousand, so the FP32 accumulation inside the MFMA
+ * stays short); wave w owns g rows [64w, 64w+64) x the 16 CT x columns of
+ * slab blockIdx.y, 4 CT accumulators of v_mfma_f32_16x16x4f32; every g
+ * element is read once per slab, x once per workgroup (the other waves hit
+ * the caches).  FP64 partial per chunk. */
+template <int CT>
+__global__ __launch_bounds__(512) void gx_f32_slab_kernel(const float* __restrict__ g,
+                                                          const float* __restrict__ x,
+                                                          double* __restrict__ part, long T,
+                                                          int NP, int ldb, long chunk) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = blockIdx.x * 64;
-    const int k0 = blockIdx.y * 64 + w * 16;
-    const long ts = (long)blockIdx.z * chunk;
+    const int r0 = w * 64;
+    const int k0 = blockIdx.y * 16 * CT;
+    const long ts = (long)blockIdx.x * chunk;
     const long te = min(T, ts + chunk);
-    f32x4 acc[4];
-    f64x4 tot[4];
+    f32x4 acc[4][CT];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        acc[a] = (f32x4){0.f, 0.f, 0.f, 0.f};
-        tot[a] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    float A[4], B[CT];
+    auto load = [&](long t4) {
+        const long t = t4 + (lane >> 4);
+        const bool ok = t < te;
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+            A[rt] = ok ? g[(size_t)t * ldb + r0 + rt * 16 + (lane & 15)] : 0.f;
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+            B[ct] = ok ? x[(size_t)t * NP + k0 + ct * 16 + (lane & 15)] : 0.f;
+    };
+    load(ts);
+    for (long t4 = ts; t4 < te; t4 += 4) {
+        float Ac[4], Bc[CT];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) Ac[rt] = A[rt];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) Bc[ct] = B[ct];
+        load(t4 + 4);
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+                acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ac[rt], Bc[ct], acc[rt][ct], 0, 0, 0);
     }
-    int steps = 0;
-    for (long t = ts; t < te; t += 4) {
-        const long tr = t + (lane >> 4);
-        const bool ok = tr < te;
-        const float bv = ok ? x[(size_t)tr * NP + k0 + (lane & 15)] : 0.f;
+    double* out = part + (size_t)blockIdx.x * ldb * NP;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const float av = ok ? g[(size_t)tr * ldb + r0 + a * 16 + (lane & 15)] : 0.f;
-            acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[a], 0, 0, 0);
-        }
-        if (++steps == 16) {
+    for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
+        for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) tot[a][r] += (double)acc[a][r];
-                acc[a] = (f32x4){0.f, 0.f, 0.f, 0.f};
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + rt * 16 + (lane >> 4) * 4 + r;
+                out[(size_t)row * NP + k0 + ct * 16 + (lane & 15)] = (double)acc[rt][ct][r];
             }
-            steps = 0;
-        }
-    }
-    double* out = part + (size_t)blockIdx.z * ldb * NP;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = r0 + a * 16 + (lane >> 4) * 4 + r;
-            out[(size_t)row * NP + k0 + (lane & 15)] = tot[a][r] + (double)acc[a][r];
-        }
 }
 
 /* ---------------- statistic GEMM G = g^T x, BF16 --------------------- */
@@ -420,12 +427,28 @@ static long gx_chunk(long T, int nchunks, int mult) {
     return ((c + mult - 1) / mult) * mult;
 }
 
+static int gx_f32_ct(int NP) { return NP % 128 == 0 ? 8 : 4; }
+
+int gx_f32_chunks(long T, int NP, int ldb, int cus) {
+    /* chunks of >= 2048 samples, at most 256 MB of partials, and at least
+     * one per CU of the budget where T allows */
+    const long by_len = (T + 2047) / 2048;
+    const long by_mem = std::max(1L, (256L << 20) / ((long)ldb * NP * 8));
+    const int slabs = NP / (16 * gx_f32_ct(NP));
+    const long want = std::max(1L, (long)cus / slabs);
+    return (int)std::max(1L, std::min(by_len, std::max(want, by_mem)));
+}
+
 int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, int ldb,
                   int nchunks, hipStream_t st) {
-    if (NP % 64 || ldb % 64) return -1;
-    dim3 grid(ldb / 64, NP / 64, nchunks);
-    hipLaunchKernelGGL(gx_f32_kernel, grid, dim3(256), 0, st, g, x, part, T, NP, ldb,
-                       gx_chunk(T, nchunks, 4));
+    if (NP % 64 || ldb % 64 || ldb > 512 || nchunks < 1) return -1;
+    const int CT = gx_f32_ct(NP);
+    dim3 grid(nchunks, NP / (16 * CT)), block(ldb);
+    const long chunk = gx_chunk(T, nchunks, 4);
+    if (CT == 8)
+        hipLaunchKernelGGL(gx_f32_slab_kernel<8>, grid, block, 0, st, g, x, part, T, NP, ldb, chunk);
+    else
+        hipLaunchKernelGGL(gx_f32_slab_kernel<4>, grid, block, 0, st, g, x, part, T, NP, ldb, chunk);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

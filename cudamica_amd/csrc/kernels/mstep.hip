@@ -16,10 +16,10 @@
  *   reparameterisation         src/amica.cu:2367-2447, kernels.cu:1023-1065
  *   W = pinv(A), ldet          src/amica.cu:27-127, 835-924
  * The reference computes pinv with a host SVD; for the non-singular A of an
- * EM run pinv(A) = A^-1, computed here by Gauss-Jordan elimination in LDS
- * (n <= 128) or LU in global memory (larger n), both with partial pivoting,
- * and log|det W| = -sum log|pivot| (the reference multiplies the diagonal
- * of the LU of W, which can overflow).
+ * EM run pinv(A) = A^-1, computed here by Gauss-Jordan elimination with
+ * partial pivoting (in LDS for n <= 128, in an L2-resident copy above), and
+ * log|det W| = -sum log|pivot| (the reference multiplies the diagonal of the
+ * LU of W, which can overflow).
  */
 #include <algorithm>
 
@@ -430,26 +430,29 @@ __device__ __forceinline__ void set_ldet(const MstepArgs& a, int h, double lad) 
     a.lconst[h] = log(a.gm[h]) + ldet + a.ds->ldetS;
 }
 
-/* n <= kLdsInvMax: W = A^-1 by Gauss-Jordan in LDS with implicit partial
- * pivoting, one workgroup per model.  Rows are never moved: step k takes
+/* W = A^-1 by Gauss-Jordan with implicit partial pivoting, one workgroup per
+ * model, the matrix in LDS (n <= kLdsInvMax) or, for larger n, in a global
+ * scratch copy (L2-resident: n n doubles per model).  Rows are never moved: step k takes
  * the pivot p_k with the largest |a_ik| among rows not pivoted yet (ties to
  * the smallest index) and eliminates in place, so that afterwards
  * a[p_r][j] = (A^-1)[r][p_j]; W is gathered from that.  The wave that
  * updates the next pivot column takes its argmax while it writes it, which
  * leaves two barriers per step. */
 constexpr int kLdsInvMax = 128;
+constexpr int kInvMax = 1024;
 
-__global__ __launch_bounds__(1024) void inverse_lds_kernel(MstepArgs a) {
-    extern __shared__ double sm[];               /* a[n n] (column-major), prow[n], pcol[n] */
-    __shared__ int piv[kLdsInvMax], pinv[kLdsInvMax], used[kLdsInvMax];
+template <bool GLOBAL>
+__global__ __launch_bounds__(1024) void inverse_gj_kernel(MstepArgs a) {
+    constexpr int NMAX = GLOBAL ? kInvMax : kLdsInvMax;
+    extern __shared__ double sm[];               /* LDS variant: a[n n] (column-major) */
+    __shared__ double prow[NMAX], pcol[NMAX];
+    __shared__ int piv[NMAX], pinv[NMAX], used[NMAX];
     __shared__ int cand_i[16];
     __shared__ double lad_sh;
     const int h = blockIdx.x, n = a.n, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int bd = blockDim.x, nw = bd >> 6;
     const long nn = (long)n * n;
-    double* aw = sm;
-    double* prow = sm + nn;
-    double* pcol = prow + n;
+    double* aw = GLOBAL ? a.work + 8 + (size_t)a.M * nn + (size_t)h * nn : sm;
     const double* A = a.A + (size_t)h * nn;
     for (long q = tid; q < nn; q += bd) aw[q] = A[q];
     for (int i = tid; i < n; i += bd) used[i] = 0;
@@ -522,17 +525,6 @@ __global__ __launch_bounds__(1024) void inverse_lds_kernel(MstepArgs a) {
     if (tid == 0) set_ldet(a, h, lad_sh);
 }
 
-/* larger n: LU with partial pivoting in global memory */
-__global__ __launch_bounds__(kBlk) void inverse_lu_kernel(MstepArgs a) {
-    __shared__ int perm[1024];
-    const int h = blockIdx.x, n = a.n;
-    const long nn = (long)n * n;
-    double* W = a.W + (size_t)h * nn;
-    double* X = a.work + 8 + (size_t)a.M * nn + (size_t)h * nn;
-    const double lad = block_lu_inverse(a.A + (size_t)h * nn, n, a.lu + (size_t)h * nn, X, W, perm);
-    if (threadIdx.x == 0) set_ldet(a, h, lad);
-}
-
 }  // namespace
 
 int launch_reduce(const double* part, int nparts, size_t pstride, const double* gxpart,
@@ -558,11 +550,11 @@ int launch_mstep(const MstepArgs& ma, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-static size_t inverse_lds_bytes(int n) { return ((size_t)n * n + 2 * (size_t)n) * sizeof(double); }
+static size_t inverse_lds_bytes(int n) { return (size_t)n * n * sizeof(double); }
 
 int mstep_configure(int n) {
     if (n > kLdsInvMax) return 0;
-    return hipFuncSetAttribute((const void*)inverse_lds_kernel,
+    return hipFuncSetAttribute((const void*)inverse_gj_kernel<false>,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)inverse_lds_bytes(n)) == hipSuccess ? 0 : -1;
 }
@@ -570,9 +562,9 @@ int mstep_configure(int n) {
 int launch_prepare(const MstepArgs& ma, hipStream_t st) {
     if (ma.n > 1024) return -1;
     if (ma.n <= kLdsInvMax)
-        hipLaunchKernelGGL(inverse_lds_kernel, dim3(ma.M), dim3(1024), inverse_lds_bytes(ma.n), st, ma);
+        hipLaunchKernelGGL(inverse_gj_kernel<false>, dim3(ma.M), dim3(1024), inverse_lds_bytes(ma.n), st, ma);
     else
-        hipLaunchKernelGGL(inverse_lu_kernel, dim3(ma.M), dim3(kBlk), 0, st, ma);
+        hipLaunchKernelGGL(inverse_gj_kernel<true>, dim3(ma.M), dim3(1024), 0, st, ma);
     const long total = (long)ma.n * ma.n + ma.n + (long)ma.n * ma.m;
     const unsigned gx = (unsigned)std::min<long>((total + 255) / 256, 256);
     hipLaunchKernelGGL(prepare_tail_kernel, dim3(gx, ma.M), dim3(256), 0, st, ma);
