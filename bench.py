@@ -34,9 +34,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+PRECISIONS = {"fp32": 32, "bf16": 16, "fp8": 8}
+PREC_NAME = {v: k for k, v in PRECISIONS.items()}
+# arithmetic types of the path: GEMM operands + density math
+DTYPE = {32: "f32", 16: "bf16+f32", 8: "fp8+bf16+f32"}
+
+
 def estep_bytes(n, T, M, precision):
     """Algorithmic HBM bytes of one E-step launch: M n T (b read + g written)."""
-    g_bytes = 2 if precision == 16 else 4
+    g_bytes = 4 if precision == 32 else 2
     return M * n * T * (4 + g_bytes)
 
 
@@ -117,7 +123,7 @@ def bench_batch(args):
     """Many recordings per EM iteration (one hipGraph): the batch workload of
     32 recordings, n in {32, 64, 128}, T ~ U[100k, 1.5M], M = 1, m = 3."""
     import cudamica_amd as ca
-    prec = 16 if args.precision == "bf16" else 32
+    prec = PRECISIONS[args.precision or "fp32"]
     ws = ca.batch_workload(precision=prec)
     t0 = time.perf_counter()
     opts = [w.options(max_iter=args.warmup + args.steps + 4) for w in ws]
@@ -145,9 +151,9 @@ def bench_batch(args):
         "value": args.steps / dt, "unit": "batched EM iters/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16+f32" if prec == 16 else "f32", "data": "synthetic",
+        "dtype": DTYPE[prec], "data": "synthetic",
         "config": {"workload": "batch32_n32-128_T100k-1.5M_M1", "recordings": len(ws),
-                   "precision": "bf16" if prec == 16 else "fp32", "parallelism": "single-gpu"},
+                   "precision": PREC_NAME[prec], "parallelism": "single-gpu"},
         "tuner_ms": dict(zip(ca.BATCH_POLICIES, [float(t) for t in ms])),
         "policy": ca.BATCH_POLICIES[p],
         "recording_iters_per_s": len(ws) * args.steps / dt,
@@ -163,7 +169,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="n128_T2M_M4",
                     help="a workload of cudamica_amd.WORKLOADS, or batch32 (32 recordings)")
-    ap.add_argument("--precision", choices=["fp32", "bf16"], default=None)
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default=None)
     ap.add_argument("--T", type=int, default=None, help="override the sample count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -181,7 +187,7 @@ def main():
     w = ca.WORKLOADS[args.workload]
     if args.T:
         w.T = args.T
-    prec = w.precision if args.precision is None else (16 if args.precision == "bf16" else 32)
+    prec = w.precision if args.precision is None else PRECISIONS[args.precision]
     n, T, M, m = w.n, w.T, w.M, w.m
 
     if args.copy_probe:
@@ -228,10 +234,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16+f32" if prec == 16 else "f32",
+        "dtype": DTYPE[prec],
         "data": "synthetic",
         "config": {"workload": w.name, "n": n, "T": T, "M": M, "m": m,
-                   "precision": "bf16" if prec == 16 else "fp32", "parallelism": "single-gpu"},
+                   "precision": PREC_NAME[prec], "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(w.name, prec),
                      "kernel": "estep_kernel", "bytes_per_launch": nbytes,

@@ -138,6 +138,7 @@ error getConfig(char* filename) {
         if ((newerr = getString(pc, (char*)"precision", &prec)) == SUCCESS) {
             if (!strcmp(prec, "fp32") || !strcmp(prec, "32")) currentConfig.precision = 32;
             else if (!strcmp(prec, "bf16") || !strcmp(prec, "16")) currentConfig.precision = 16;
+            else if (!strcmp(prec, "fp8") || !strcmp(prec, "8")) currentConfig.precision = 8;
             else {
                 fprintf(stderr, "ERROR: Invalid precision (precision): %s\n", prec);
                 retorno = ERRORINVALIDPARAM;
@@ -198,7 +199,8 @@ void printConfig(void) {
     fprintf(stdout, "do_newton %lu\n", currentConfig.do_newton);
     fprintf(stdout, "fix_init %lu\n", currentConfig.fix_init);
     fprintf(stdout, "block_size %lu\n", currentConfig.block_size);
-    fprintf(stdout, "precision %s\n", currentConfig.precision == 16 ? "bf16" : "fp32");
+    fprintf(stdout, "precision %s\n", currentConfig.precision == 16 ? "bf16"
+                                      : currentConfig.precision == 8 ? "fp8" : "fp32");
     fprintf(stdout, "====================================\n\n");
 }
 

@@ -79,6 +79,9 @@ struct amica_engine {
     double* xw64 = nullptr;        /* FP64 whitened data (FP32 path) */
     uint16_t* xbf = nullptr;       /* bf16 whitened data, hi then lo part (BF16 path) */
     uint16_t* x8 = nullptr;        /* bf16 hi part packed 8 samples per column (BF16 path) */
+    int fp8 = 0;                   /* FP8 storage of the whitened data */
+    uint8_t* xq = nullptr;         /* FP8 e4m3 data, [T][NP] */
+    float* xscale = nullptr;       /* per-channel FP8 scales, [NP] */
     float* b = nullptr;            /* FP32 unmixed data (BF16 path) */
     double* b64 = nullptr;         /* FP64 unmixed data (FP32 path) */
     void* g = nullptr;
@@ -163,6 +166,7 @@ struct amica_engine {
         a.A = A; a.c = c; a.alpha = alpha; a.mu = mu; a.beta = beta; a.rho = rho; a.gm = gm;
         a.W = W; a.ldet = ldet; a.lconst = lconst;
         a.Wg64 = Wg64; a.wc64 = wc64; a.Whi = Whi; a.Wlo = Wlo; a.wc = wc;
+        a.xscale = fp8 ? xscale : nullptr;
         a.cst = cst; a.cst64 = cst64;
         a.stats = stats; a.stride = stride;
         a.sigma2 = sigma2; a.cnew = cnew; a.phis = phis; a.kappa = kappa; a.lambda = lambda;
@@ -189,7 +193,10 @@ struct amica_engine {
         if (with_timing) HIPCHK(hipEventRecord(ev[0], st));
         LCHK(launch_prepare(ma, st));
         if (bf16)
-            LCHK(launch_unmix_bf16x3(xw, xbf, Whi, Wlo, wc, b, o.T, NP, ldb, cu_budget, st));
+        {
+            if (fp8) LCHK(launch_unmix_fp8(xq, Whi, Wlo, wc, b, o.T, NP, ldb, cu_budget, st));
+            else LCHK(launch_unmix_bf16x3(xw, xbf, Whi, Wlo, wc, b, o.T, NP, ldb, cu_budget, st));
+        }
         else
             LCHK(launch_unmix_f64(xw64, Wg64, wc64, b64, o.T, NP, ldb, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
@@ -248,8 +255,10 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     if (o.T < 2) return fail(AMICA_E_INVALID, "T must be >= 2");
     if (o.M < 1 || o.M > AMICA_MAX_M) return fail(AMICA_E_INVALID, "M must be in [1, 16]");
     if (o.m < 1 || o.m > AMICA_MAX_MJ) return fail(AMICA_E_INVALID, "m must be in [1, 8]");
-    if (o.precision != AMICA_PREC_FP32 && o.precision != AMICA_PREC_BF16)
-        return fail(AMICA_E_INVALID, "precision must be 32 (fp32) or 16 (bf16)");
+    if (o.precision != AMICA_PREC_FP32 && o.precision != AMICA_PREC_BF16 && o.precision != AMICA_PREC_FP8)
+        return fail(AMICA_E_INVALID, "precision must be 32 (fp32), 16 (bf16) or 8 (fp8)");
+    if (o.precision == AMICA_PREC_FP8 && ((o.n + 63) / 64) * 64 > 256)
+        return fail(AMICA_E_INVALID, "fp8 storage needs n <= 256");
     const int NP = ((o.n + 63) / 64) * 64;
     const int CW = NP / 64;
     if (o.M * CW > kMaxEstepWaves)
@@ -265,7 +274,8 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     e->NP = NP;
     e->CW = CW;
     e->ldb = o.M * NP;
-    e->bf16 = (o.precision == AMICA_PREC_BF16);
+    e->bf16 = (o.precision == AMICA_PREC_BF16 || o.precision == AMICA_PREC_FP8);
+    e->fp8 = (o.precision == AMICA_PREC_FP8);
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, o.device));
     e->cus = prop.multiProcessorCount;
@@ -283,7 +293,11 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     };
 #define ALLOC(ptr, bytes) do { int _r = alloc((void**)&(ptr), (bytes)); if (_r) { amica_engine_destroy(e); return _r; } } while (0)
     ALLOC(e->xw, (size_t)T * NP * sizeof(float));
-    if (e->bf16) {
+    if (e->fp8) {
+        ALLOC(e->xq, (size_t)T * NP);
+        ALLOC(e->xscale, (size_t)NP * sizeof(float));
+        ALLOC(e->b, (size_t)T * e->ldb * sizeof(float));
+    } else if (e->bf16) {
         ALLOC(e->xbf, 2 * (size_t)T * NP * sizeof(uint16_t));   /* hi, then lo */
         ALLOC(e->b, (size_t)T * e->ldb * sizeof(float));
     } else {
@@ -385,7 +399,7 @@ int amica_engine_destroy(amica_engine* e) {
     if (e->st) (void)hipStreamSynchronize(e->st);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
-    void* ptrs[] = {e->xraw, e->xw, e->xw64, e->xbf, e->x8, e->b, e->b64, e->g, e->part, e->gxpart,
+    void* ptrs[] = {e->xraw, e->xw, e->xw64, e->xbf, e->x8, e->xq, e->xscale, e->b, e->b64, e->g, e->part, e->gxpart,
                     e->raw, e->gxtot, e->A, e->c, e->alpha, e->mu, e->beta, e->rho, e->gm, e->W,
                     e->ldet, e->lconst, e->Wg64, e->wc64, e->Whi, e->Wlo, e->wc, e->cst,
                     e->cst64, e->stats, e->sigma2, e->cnew, e->phis,
@@ -440,7 +454,9 @@ int amica_engine_whiten(amica_engine* e, int do_sphere) {
     const long T = e->o.T;
     const int nchunks = (int)std::max(1L, std::min((long)e->cus * 2, T / 64 + 1));
     double *part = nullptr, *C = nullptr, *V = nullptr, *d = nullptr;
-    const size_t partn = std::max((size_t)nchunks * n * n, (size_t)nchunks * 2 * n);
+    /* scratch of the covariance and sphering partials, and of the FP8 channel maxima */
+    const size_t partn = std::max(std::max((size_t)nchunks * n * n, (size_t)nchunks * 2 * n),
+                                  (size_t)nchunks * e->NP);
     HIPCHK(hipMalloc((void**)&part, partn * sizeof(double)));
     HIPCHK(hipMalloc((void**)&C, (size_t)n * n * sizeof(double)));
     HIPCHK(hipMalloc((void**)&V, (size_t)n * n * sizeof(double)));
@@ -454,7 +470,9 @@ int amica_engine_whiten(amica_engine* e, int do_sphere) {
         if (launch_sphere_matrix(V, d, C, n, e->xraw, e->mean, T, do_sphere, e->S, e->ldetS_d,
                                  part, nchunks, e->st)) { rc = -1; break; }
         if (launch_whiten(e->xraw, e->mean, e->S, T, n, e->NP, e->xw, e->xw64, e->xbf, e->st)) { rc = -1; break; }
-        if (e->x8 && launch_pack8(e->xbf, T, e->NP, e->x8, e->st)) { rc = -1; break; }
+        if (e->x8 && !e->fp8 && launch_pack8(e->xbf, T, e->NP, e->x8, e->st)) { rc = -1; break; }
+        if (e->fp8 && launch_quantize_fp8(e->xw, T, e->NP, e->xscale, e->xq, e->x8, (float*)part,
+                                          nchunks, e->st)) { rc = -1; break; }
     } while (0);
     const hipError_t se = hipStreamSynchronize(e->st);
     (void)hipFree(part); (void)hipFree(C); (void)hipFree(V); (void)hipFree(d);
@@ -489,9 +507,17 @@ int amica_engine_load_white(amica_engine* e, const double* xw, double ldetS) {
     HIPCHK(hipMalloc((void**)&tmp, cnt * sizeof(double)));
     HIPCHK(hipMemcpy(tmp, xw, cnt * sizeof(double), hipMemcpyHostToDevice));
     int rc = launch_white_from_f64(tmp, e->o.T, e->o.n, e->NP, e->xw, e->xw64, e->xbf, e->st);
-    if (!rc && e->x8) rc = launch_pack8(e->xbf, e->o.T, e->NP, e->x8, e->st);
+    float* qpart = nullptr;
+    const int qchunks = (int)std::max(1L, std::min((long)e->cus * 2, e->o.T / 64 + 1));
+    if (!rc && e->fp8) {
+        if (hipMalloc((void**)&qpart, (size_t)qchunks * e->NP * sizeof(float)) != hipSuccess) rc = -1;
+        else rc = launch_quantize_fp8(e->xw, e->o.T, e->NP, e->xscale, e->xq, e->x8, qpart, qchunks, e->st);
+    } else if (!rc && e->x8) {
+        rc = launch_pack8(e->xbf, e->o.T, e->NP, e->x8, e->st);
+    }
     hipError_t se = hipStreamSynchronize(e->st);
     (void)hipFree(tmp);
+    if (qpart) (void)hipFree(qpart);
     if (rc || se != hipSuccess) return fail(AMICA_E_DEVICE, "load_white failed");
     e->ldetS = ldetS;
     /* S and mean are unknown for externally whitened data: identity, zero */

@@ -79,6 +79,10 @@ int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double
 int launch_unmix_bf16x3(const float* x, const uint16_t* xbf, const uint16_t* Whi,
                         const uint16_t* Wlo, const float* wc, float* b, long T, int NP, int ldb,
                         int cus, hipStream_t st);
+/* FP8 path: b = q W_hi + q W_lo with q the FP8 e4m3 data [T][NP] and W~ the
+ * split of W diag(scale) (NP <= 256). */
+int launch_unmix_fp8(const uint8_t* xq, const uint16_t* Whi, const uint16_t* Wlo, const float* wc,
+                     float* b, long T, int NP, int ldb, int cus, hipStream_t st);
 /* part[chunk][c][k] = sum_{t in chunk} g[t][c] x[t][k]   (FP64 partials) */
 int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, int ldb,
                   int nchunks, hipStream_t st);
@@ -121,6 +125,7 @@ struct MstepArgs {
      * BF16 path Whi + Wlo (bf16) + wc */
     double* Wg64;          /* [ldb][NP] */
     double* wc64;          /* [ldb] */
+    const float* xscale;   /* [NP] per-channel FP8 scales folded into Whi/Wlo, or null */
     uint16_t* Whi;         /* [ldb][NP] */
     uint16_t* Wlo;         /* [ldb][NP] */
     float* wc;             /* [ldb] */
@@ -166,6 +171,11 @@ int launch_whiten(const float* x, const double* mean, const double* S, long T, i
 int launch_white_from_f64(const double* xin, long T, int n, int NP, float* xw, double* xw64,
                           uint16_t* xbf, hipStream_t st);
 int launch_bf16_from_f32(const float* x, size_t count, uint16_t* out, hipStream_t st);
+/* FP8 storage: per-channel scales (part: nchunks x NP floats of scratch),
+ * xq = e4m3(x / scale), x <- scale dequant(xq), x8 = bf16(dequant(xq))
+ * packed 8 samples per channel. */
+int launch_quantize_fp8(float* xw, long T, int NP, float* scale, uint8_t* xq, uint16_t* x8,
+                        float* part, int nchunks, hipStream_t st);
 /* Sinv = S^-1 (LU), A_out = colnorm(Sinv A_h), c_out = Sinv c_h + mean. */
 int launch_output_transform(const double* S, const double* mean, const double* A,
                             const double* c, int n, int M, double* lu, double* Sinv,

@@ -163,27 +163,48 @@ __global__ __launch_bounds__(256) void unmix_bf16x3_kernel(const float* __restri
     }
 }
 
-/* Register-resident split-BF16 unmixing.  x is pre-split (xhi = bf16(x),
- * xlo = bf16(x - xhi), written once per recording by the whitening kernel),
- * so the product needs no conversions.  Wave w of a workgroup owns columns
- * [16 CT w, 16 CT (w+1)) and keeps their W_hi / W_lo B-fragments for all
- * NP = 32 KS <= 128 inputs in registers (CT KS 8 VGPRs) for the whole launch; the
- * waves of a workgroup walk the same 16-sample tiles (tile = blockIdx.x,
- * += gridDim.x), so each x tile comes from HBM once and from L1/L2 for the
- * other waves, and the A fragments of the next tile are loaded while the
- * current tile's 3 CT KS MFMAs run.  HBM traffic per launch: x hi+lo
- * (4 NP bytes per sample) + b (4 ldb bytes per sample). */
-template <int CT, int KS>
-__global__ __launch_bounds__(512) void unmix_bf16x3_reg_kernel(const uint16_t* __restrict__ xhi,
-                                                                const uint16_t* __restrict__ xlo,
-                                                                const uint16_t* __restrict__ Whi,
-                                                                const uint16_t* __restrict__ Wlo,
-                                                                const float* __restrict__ wc,
-                                                                float* __restrict__ b, long T,
-                                                                int ldb, long ntiles) {
+/* 8 FP8 e4m3 values (two dwords) -> 8 bf16, exactly (e4m3 has 3 mantissa
+ * bits; the FP32 value the hardware conversion gives is truncated to bf16
+ * without loss). */
+__device__ __forceinline__ i16x8 fp8x8_to_bf16x8(uint2 q) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 v0 = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.x, false);
+    const f2 v1 = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.x, true);
+    const f2 v2 = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.y, false);
+    const f2 v3 = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.y, true);
+    i16x8 r;
+    r[0] = (short)(__float_as_uint(v0.x) >> 16); r[1] = (short)(__float_as_uint(v0.y) >> 16);
+    r[2] = (short)(__float_as_uint(v1.x) >> 16); r[3] = (short)(__float_as_uint(v1.y) >> 16);
+    r[4] = (short)(__float_as_uint(v2.x) >> 16); r[5] = (short)(__float_as_uint(v2.y) >> 16);
+    r[6] = (short)(__float_as_uint(v3.x) >> 16); r[7] = (short)(__float_as_uint(v3.y) >> 16);
+    return r;
+}
+
+/* Register-resident unmixing, b = x W~^T - W~ c with W~ = W_hi + W_lo (bf16):
+ *   Q8 = false: x pre-split into bf16 hi and lo (written once per recording
+ *     by the whitening kernel), b = x_hi W_hi + x_hi W_lo + x_lo W_hi
+ *     (~2^-16 relative);
+ *   Q8 = true: x stored as FP8 e4m3 q with the per-channel scales folded
+ *     into W (W~ = split(W diag(s))), b = q W_hi + q W_lo, q exact in bf16.
+ * A workgroup of waves covers one column group (blockIdx.y); wave w owns
+ * 16 CT columns and keeps their W_hi / W_lo B-fragments for all NP = 32 KS
+ * inputs in registers for the whole launch.  The waves of a workgroup walk
+ * the same 16-sample tiles (tile = blockIdx.x, += gridDim.x), so each x tile
+ * comes from HBM once per column group and from L1/L2 for the other waves;
+ * the next tile's A fragments load while the current tile's MFMAs run.
+ * HBM traffic per launch: x (4 NP or NP bytes per sample, per column group)
+ * + b (4 ldb bytes per sample). */
+template <int CT, int KS, bool Q8>
+__global__ __launch_bounds__(512) void unmix_reg_kernel(const void* __restrict__ xa,
+                                                         const uint16_t* __restrict__ xlo,
+                                                         const uint16_t* __restrict__ Whi,
+                                                         const uint16_t* __restrict__ Wlo,
+                                                         const float* __restrict__ wc,
+                                                         float* __restrict__ b, long T, int ldb,
+                                                         long ntiles) {
     constexpr int NP = 32 * KS;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int c0 = w * 16 * CT;
+    const int c0 = (blockIdx.y * (blockDim.x >> 6) + w) * 16 * CT;
     const int kofs = (lane >> 4) * 8;
     i16x8 bhi[CT][KS], blo[CT][KS];
     float bias[CT];
@@ -198,23 +219,36 @@ __global__ __launch_bounds__(512) void unmix_bf16x3_reg_kernel(const uint16_t* _
             blo[c][k] = *(const i16x8*)(Wlo + wofs);
         }
     }
-    i16x8 ahi[KS], alo[KS];
+    constexpr int NA = Q8 ? 1 : 2;          /* A operands per k-step */
+    i16x8 an[NA][KS];                        /* bf16 path: loaded A; FP8 path unused */
+    uint2 qn[KS];                            /* FP8 path: loaded bytes */
     auto load_a = [&](long tile) {
         const long t = tile * 16 + (lane & 15);
         const bool ok = tile < ntiles && t < T;
 #pragma unroll
         for (int k = 0; k < KS; ++k) {
             const size_t xofs = (size_t)t * NP + k * 32 + kofs;
-            ahi[k] = ok ? *(const i16x8*)(xhi + xofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
-            alo[k] = ok ? *(const i16x8*)(xlo + xofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
+            if constexpr (Q8) {
+                qn[k] = ok ? *(const uint2*)((const uint8_t*)xa + xofs) : make_uint2(0u, 0u);
+            } else {
+                an[0][k] = ok ? *(const i16x8*)((const uint16_t*)xa + xofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
+                an[1][k] = ok ? *(const i16x8*)(xlo + xofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
+            }
         }
     };
     long tile = blockIdx.x;
     load_a(tile);
     for (; tile < ntiles; tile += gridDim.x) {
-        i16x8 chi[KS], clo[KS];
+        i16x8 ahi[KS], alo[KS];
 #pragma unroll
-        for (int k = 0; k < KS; ++k) { chi[k] = ahi[k]; clo[k] = alo[k]; }
+        for (int k = 0; k < KS; ++k) {
+            if constexpr (Q8) {
+                ahi[k] = fp8x8_to_bf16x8(qn[k]);
+            } else {
+                ahi[k] = an[0][k];
+                alo[k] = an[1][k];
+            }
+        }
         load_a(tile + gridDim.x);
         f32x4 acc[CT];
 #pragma unroll
@@ -223,11 +257,12 @@ __global__ __launch_bounds__(512) void unmix_bf16x3_reg_kernel(const uint16_t* _
         for (int k = 0; k < KS; ++k)
 #pragma unroll
             for (int c = 0; c < CT; ++c) {
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, clo[k]),
-                                                                 __builtin_bit_cast(bf16x8, bhi[c][k]), acc[c], 0, 0, 0);
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, chi[k]),
+                if constexpr (!Q8)
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, alo[k]),
+                                                                     __builtin_bit_cast(bf16x8, bhi[c][k]), acc[c], 0, 0, 0);
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ahi[k]),
                                                                  __builtin_bit_cast(bf16x8, blo[c][k]), acc[c], 0, 0, 0);
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, chi[k]),
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ahi[k]),
                                                                  __builtin_bit_cast(bf16x8, bhi[c][k]), acc[c], 0, 0, 0);
             }
 #pragma unroll
@@ -394,32 +429,62 @@ int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+/* Column tiles per wave and k-steps of the register-resident kernels, or
+ * CT = 0 where none fits (B fragments: 4 CT KS VGPRs per bf16 part). */
+static void unmix_shape(int NP, bool q8, int& CT, int& KS) {
+    KS = NP / 32;
+    switch (NP) {
+        case 64: CT = 4; break;
+        case 128: CT = 4; break;
+        case 192: CT = 2; break;
+        case 256: CT = q8 ? 2 : 1; break;
+        default: CT = 0; break;
+    }
+}
+
+template <bool Q8>
+static int launch_unmix_reg(const void* xa, const uint16_t* xlo, const uint16_t* Whi,
+                            const uint16_t* Wlo, const float* wc, float* b, long T, int NP, int ldb,
+                            int cus, hipStream_t st) {
+    int CT, KS;
+    unmix_shape(NP, Q8, CT, KS);
+    if (CT == 0 || ldb % (16 * CT)) return -1;
+    /* waves per workgroup: the largest divisor of the wave count <= 8 */
+    const int nwaves = ldb / (16 * CT);
+    int waves = 8;
+    while (nwaves % waves) --waves;
+    const int groups = nwaves / waves;
+    const long ntiles = (T + 15) / 16;
+    const long gx = std::max(1L, std::min<long>(ntiles, (long)cus * std::max(1, 8 / waves) / groups));
+    const dim3 g((unsigned)gx, groups), blk(64 * waves);
+#define AMICA_UNMIX_REG(C, K)                                                                   \
+    hipLaunchKernelGGL((unmix_reg_kernel<C, K, Q8>), g, blk, 0, st, xa, xlo, Whi, Wlo, wc, b,  \
+                       T, ldb, ntiles)
+    if (CT == 4 && KS == 2) AMICA_UNMIX_REG(4, 2);
+    else if (CT == 4 && KS == 4) AMICA_UNMIX_REG(4, 4);
+    else if (CT == 2 && KS == 6) AMICA_UNMIX_REG(2, 6);
+    else if (CT == 2 && KS == 8) AMICA_UNMIX_REG(2, 8);
+    else if (CT == 1 && KS == 8) AMICA_UNMIX_REG(1, 8);
+    else return -1;
+#undef AMICA_UNMIX_REG
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_unmix_bf16x3(const float* x, const uint16_t* xbf, const uint16_t* Whi,
                         const uint16_t* Wlo, const float* wc, float* b, long T, int NP, int ldb,
                         int cus, hipStream_t st) {
     if (NP % 32 || ldb % 64) return -1;
-    /* register-resident W: 64 columns per wave, NP <= 128, ldb <= 512 */
-    constexpr int CT = 4;
-    const int waves = ldb / (16 * CT);
-    if (xbf && NP <= 128 && waves <= 8) {
-        const long ntiles = (T + 15) / 16;
-        const long grid = std::min<long>(ntiles, (long)cus * std::max(1, 8 / waves));
-        const uint16_t* xlo = xbf + (size_t)T * NP;
-        const dim3 g((unsigned)grid), blk(64 * waves);
-#define AMICA_UNMIX_REG(C, K)                                                                   \
-        hipLaunchKernelGGL((unmix_bf16x3_reg_kernel<C, K>), g, blk, 0, st, xbf, xlo, Whi, Wlo,   \
-                           wc, b, T, ldb, ntiles)
-        switch (NP) {
-            case 64: AMICA_UNMIX_REG(4, 2); break;
-            case 128: AMICA_UNMIX_REG(4, 4); break;
-            default: return -1;
-        }
-#undef AMICA_UNMIX_REG
-        return hipGetLastError() == hipSuccess ? 0 : -1;
-    }
+    if (xbf && launch_unmix_reg<false>(xbf, xbf + (size_t)T * NP, Whi, Wlo, wc, b, T, NP, ldb, cus, st) == 0)
+        return 0;
     dim3 grid((unsigned)((T + 63) / 64), ldb / 64);
     hipLaunchKernelGGL(unmix_bf16x3_kernel, grid, dim3(256), 0, st, x, Whi, Wlo, wc, b, T, NP, ldb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_unmix_fp8(const uint8_t* xq, const uint16_t* Whi, const uint16_t* Wlo, const float* wc,
+                     float* b, long T, int NP, int ldb, int cus, hipStream_t st) {
+    if (NP % 32 || ldb % 64) return -1;
+    return launch_unmix_reg<true>(xq, nullptr, Whi, Wlo, wc, b, T, NP, ldb, cus, st);
 }
 
 static long gx_chunk(long T, int nchunks, int mult) {
@@ -430,9 +495,9 @@ static long gx_chunk(long T, int nchunks, int mult) {
 static int gx_f32_ct(int NP) { return NP % 128 == 0 ? 8 : 4; }
 
 int gx_f32_chunks(long T, int NP, int ldb, int cus) {
-    /* chunks of >= 2048 samples, at most 256 MB of partials, and at least
+    /* chunks of >= 512 samples, at most 256 MB of partials, and at least
      * one per CU of the budget where T allows */
-    const long by_len = (T + 2047) / 2048;
+    const long by_len = (T + 511) / 512;
     const long by_mem = std::max(1L, (256L << 20) / ((long)ldb * NP * 8));
     const int slabs = NP / (16 * gx_f32_ct(NP));
     const long want = std::max(1L, (long)cus / slabs);

@@ -379,8 +379,9 @@ __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
             const double wv2 = W[q];
             const size_t dst = ((size_t)h * NP + i) * NP + k;
             if (a.prec_bf16) {
+                /* FP8 storage: the GEMM multiplies q = x / s, so it gets W diag(s) */
                 uint16_t hi, lo;
-                split_w(wv2, hi, lo);
+                split_w(a.xscale ? wv2 * (double)a.xscale[k] : wv2, hi, lo);
                 a.Whi[dst] = hi;
                 a.Wlo[dst] = lo;
             } else {
@@ -393,9 +394,11 @@ __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
             for (int k = 0; k < n; ++k) {
                 double wr = W[i + (long)k * n];
                 if (a.prec_bf16) {
+                    /* the W~ the unmixing GEMM applies (W~ = split(W diag(s)) diag(1/s)) */
+                    const double sk = a.xscale ? (double)a.xscale[k] : 1.0;
                     uint16_t hi, lo;
-                    split_w(wr, hi, lo);
-                    wr = joined_w(hi, lo);
+                    split_w(wr * sk, hi, lo);
+                    wr = joined_w(hi, lo) / sk;
                 }
                 s += wr * c[k];
             }

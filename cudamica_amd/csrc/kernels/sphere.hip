@@ -304,6 +304,47 @@ __global__ void white_from_f64_kernel(const double* __restrict__ xin, long T, in
     if (xbf) store_split_bf16(xbf, (size_t)T * NP, e, f);
 }
 
+/* FP8 storage of the whitened data (per-channel scales s_i = max_t |x_ti| /
+ * 448, the largest finite e4m3 value): colmaxabs over sample chunks, then
+ * the scales (order-free max: deterministic), then per element q =
+ * e4m3(x / s) (hardware conversion, round to nearest even), the data the
+ * algorithm runs on x^ = s dequant(q) written back into xw, and bf16(q)
+ * (exact) packed 8 samples per channel for the statistic GEMM. */
+__global__ __launch_bounds__(256) void colmaxabs_kernel(const float* __restrict__ xw, long T, int NP,
+                                                         long chunk, float* __restrict__ part) {
+    const int i = blockIdx.y * 256 + threadIdx.x;
+    if (i >= NP) return;
+    const long t0 = (long)blockIdx.x * chunk, t1 = min(T, t0 + chunk);
+    float mx = 0.f;
+    for (long t = t0; t < t1; ++t) mx = fmaxf(mx, fabsf(xw[(size_t)t * NP + i]));
+    part[(size_t)blockIdx.x * NP + i] = mx;
+}
+
+__global__ void fp8_scale_kernel(const float* __restrict__ part, int nchunks, int NP,
+                                 float* __restrict__ scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NP) return;
+    float mx = 0.f;
+    for (int c = 0; c < nchunks; ++c) mx = fmaxf(mx, part[(size_t)c * NP + i]);
+    scale[i] = mx > 0.f ? mx / 448.f : 1.f;
+}
+
+__global__ void quantize_fp8_kernel(float* __restrict__ xw, long T, int NP,
+                                    const float* __restrict__ scale, uint8_t* __restrict__ xq,
+                                    uint16_t* __restrict__ x8) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)T * NP) return;
+    const long t = (long)(e / NP);
+    const int i = (int)(e % NP);
+    const float s = scale[i];
+    const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(xw[e] / s, 0.f, 0, false);
+    const f2 d = __builtin_amdgcn_cvt_pk_f32_fp8(packed, false);
+    xq[e] = (uint8_t)(packed & 0xff);
+    xw[e] = d.x * s;
+    x8[((size_t)(t >> 3) * NP + i) * 8 + (t & 7)] = (uint16_t)(__float_as_uint(d.x) >> 16);
+}
+
 __global__ void bf16_from_f32_kernel(const float* __restrict__ x, size_t count,
                                      uint16_t* __restrict__ out) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -411,6 +452,19 @@ int launch_white_from_f64(const double* xin, long T, int n, int NP, float* xw, d
     const size_t cnt = (size_t)T * NP;
     hipLaunchKernelGGL(white_from_f64_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0,
                        st, xin, T, n, NP, xw, xw64, xbf);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_quantize_fp8(float* xw, long T, int NP, float* scale, uint8_t* xq, uint16_t* x8,
+                        float* part, int nchunks, hipStream_t st) {
+    const long chunk = (T + nchunks - 1) / nchunks;
+    hipLaunchKernelGGL(colmaxabs_kernel, dim3(nchunks, (NP + 255) / 256), dim3(256), 0, st, xw, T, NP,
+                       chunk, part);
+    hipLaunchKernelGGL(fp8_scale_kernel, dim3((NP + 255) / 256), dim3(256), 0, st, part, nchunks, NP,
+                       scale);
+    const size_t cnt = (size_t)T * NP;
+    hipLaunchKernelGGL(quantize_fp8_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, xw,
+                       T, NP, scale, xq, x8);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

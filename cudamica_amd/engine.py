@@ -15,6 +15,7 @@ CLI_PATH = os.path.join(_HERE, "bin", "cudaamica")
 
 PREC_FP32 = 32
 PREC_BF16 = 16
+PREC_FP8 = 8
 
 
 class Options(ctypes.Structure):

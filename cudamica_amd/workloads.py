@@ -61,10 +61,11 @@ WORKLOADS = {
     "n128_T2M_M4": Workload("n128_T2M_M4", n=128, T=2_000_000, M=4, m=3, iters=200,
                             seed=0x5EED0002, sub_frac=0.1, nmix=4, block=50_000, alternate=0,
                             precision=_e.PREC_BF16),
-    # 256 ch x 4M samples, 2 models alternating in 100k blocks, 4 components
+    # 256 ch x 4M samples, 2 models alternating in 100k blocks, 4 components,
+    # whitened data stored as FP8 e4m3 with per-channel scales
     "n256_T4M_M2": Workload("n256_T4M_M2", n=256, T=4_000_000, M=2, m=4, iters=200,
                             seed=0x5EED0004, nmix=2, block=100_000, alternate=1, cond_max=0.0,
-                            precision=_e.PREC_BF16),
+                            precision=_e.PREC_FP8),
 }
 
 

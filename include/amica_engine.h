@@ -34,6 +34,9 @@ extern "C" {
 
 #define AMICA_PREC_FP32       32    /* FP32 GEMMs and density math */
 #define AMICA_PREC_BF16       16    /* BF16 GEMM operands, FP32 accumulation and density math */
+#define AMICA_PREC_FP8         8    /* as BF16, with the whitened data stored as FP8 e4m3 with
+                                       per-channel scales (the algorithm then runs on the
+                                       dequantized data) */
 
 typedef struct amica_engine amica_engine;
 
@@ -42,7 +45,7 @@ typedef struct {
     long   T;                /* samples */
     int    M;                /* models */
     int    m;                /* mixture components per source (1..8) */
-    int    precision;        /* AMICA_PREC_FP32 or AMICA_PREC_BF16 */
+    int    precision;        /* AMICA_PREC_FP32, AMICA_PREC_BF16 or AMICA_PREC_FP8 */
     int    max_iter;         /* capacity of the per-iteration logs */
     int    device;           /* HIP device ordinal */
     /* EM options, same meaning and defaults as the reference (config.h:46-83) */

@@ -122,6 +122,8 @@ def test_devices_list_and_precision(cfg, tmp_path):
     assert rc == 0 and c.devcount == 3 and list(c.devs[:3]) == [2, 0, 1]
     assert c.precision == 16 and c.seed == 99
     rc, c = cfg(base + "precision fp8\n", str(tmp_path))
+    assert rc == 0 and c.precision == 8
+    rc, c = cfg(base + "precision fp4\n", str(tmp_path))
     assert rc == -3
 
 
