@@ -46,11 +46,16 @@ def _run(cmd):
     return out
 
 
+# the Gauss-Jordan panel steps are unrolled in full so that the panel stays
+# in registers with static indices
+FILE_FLAGS = {"kernels/mstep.hip": ["-mllvm", "-pragma-unroll-threshold=262144"]}
+
+
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + FILE_FLAGS.get(src, []) + ["-c", path, "-o", obj]
     else:
         cmd = [GXX] + CXX_FLAGS + ["-c", path, "-o", obj]
     _run(cmd)

@@ -373,7 +373,7 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     ALLOC(e->kappa, M * (size_t)n * sizeof(double));
     ALLOC(e->lambda, M * (size_t)n * sizeof(double));
     ALLOC(e->Bm, M * nn * sizeof(double));
-    ALLOC(e->work, (8 + 2 * M * nn) * sizeof(double));
+    ALLOC(e->work, (8 + M * nn + inverse_work_doubles(n, M)) * sizeof(double));
     ALLOC(e->lu, std::max((size_t)M, (size_t)2) * nn * sizeof(double));
     ALLOC(e->bflag, M * sizeof(int));
     ALLOC(e->newton, M * sizeof(int));
@@ -743,6 +743,7 @@ int amica_engine_debug_buffer(amica_engine* e, const char* name, void* out, size
         {"Wg64", e->Wg64, e->Wg64 ? ldb * NP * sizeof(double) : 0},
         {"wc64", e->wc64, e->wc64 ? ldb * sizeof(double) : 0},
         {"wc", e->wc, ldb * sizeof(float)},
+        {"ldet", e->ldet, (size_t)e->o.M * sizeof(double)},
     };
     for (const Buf& b : bufs) {
         if (std::strcmp(b.n, name) != 0) continue;

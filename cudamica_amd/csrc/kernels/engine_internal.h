@@ -153,6 +153,8 @@ int launch_mstep(const MstepArgs& ma, hipStream_t st);
 int launch_prepare(const MstepArgs& ma, hipStream_t st);
 /* Allows the LDS inverse its dynamic shared memory (once per engine). */
 int mstep_configure(int n);
+/* doubles of M-step scratch the inverse uses after the first 8 + M n n */
+size_t inverse_work_doubles(int n, int M);
 
 /* Preprocessing -------------------------------------------------------------- */
 int launch_colmean(const float* x, long T, int n, double* part, int nchunks, double* mean,

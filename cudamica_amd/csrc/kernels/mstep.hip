@@ -433,99 +433,322 @@ __device__ __forceinline__ void set_ldet(const MstepArgs& a, int h, double lad) 
     a.lconst[h] = log(a.gm[h]) + ldet + a.ds->ldetS;
 }
 
-/* W = A^-1 by Gauss-Jordan with implicit partial pivoting, one workgroup per
- * model, the matrix in LDS (n <= kLdsInvMax) or, for larger n, in a global
- * scratch copy (L2-resident: n n doubles per model).  Rows are never moved: step k takes
- * the pivot p_k with the largest |a_ik| among rows not pivoted yet (ties to
- * the smallest index) and eliminates in place, so that afterwards
- * a[p_r][j] = (A^-1)[r][p_j]; W is gathered from that.  The wave that
- * updates the next pivot column takes its argmax while it writes it, which
- * leaves two barriers per step. */
-constexpr int kLdsInvMax = 128;
+/* W = A^-1 by blocked Gauss-Jordan with implicit partial pivoting.
+ *
+ * Rows are never moved: step k takes the pivot p_k with the largest |a_ik|
+ * among the rows not pivoted yet (ties to the smallest index) and eliminates
+ * in place,
+ *   a[p][j] <- a[p][j] / d (j != k),   a[p][k] <- 1 / d,
+ *   a[i][j] <- a[i][j] - a[i][k] a[p][j] / d,   a[i][k] <- -a[i][k] / d,
+ * with d = a[p][k], so that afterwards a[p_r][j] = (A^-1)[r][p_j]; W is
+ * gathered from that and log|det A| = sum_k log|d_k|.
+ *
+ * Blocked over panels of NB columns K = [k0, k0 + NB) with pivot rows P:
+ * NB steps restricted to the panel turn its columns into A_PK^-1 (rows P)
+ * and -A_RK A_PK^-1 (the other rows R), and the remaining columns J then
+ * take all NB steps at once,
+ *   a'[i][j] = (i in P ? 0 : a[i][j]) + sum_q a'[i][k0 + q] a[p_q][j],
+ * which is the Schur complement form of the same elimination.  A panel is
+ * factored by one wave with its n x NB values in registers (lane l holds
+ * rows l + 64 r), so a step costs no barrier beyond the pivot row's trip
+ * through LDS; the trailing update runs one wave per column over the whole
+ * GPU.  Every panel after the first is factored by block 0 of the launch
+ * that applies the previous panel, right after it has updated the new
+ * panel's columns.
+ *
+ * Scratch per model h: the n x n matrix (column-major) at
+ * work + 8 + M n n + h n n, then the pivot bookkeeping after all matrices:
+ * step[n] (the step that pivoted row i, -1 while unused) and piv[n] (the
+ * pivot row of step k) as ints, then |d_k| per step. */
 constexpr int kInvMax = 1024;
 
-template <bool GLOBAL>
-__global__ __launch_bounds__(1024) void inverse_gj_kernel(MstepArgs a) {
-    constexpr int NMAX = GLOBAL ? kInvMax : kLdsInvMax;
-    extern __shared__ double sm[];               /* LDS variant: a[n n] (column-major) */
-    __shared__ double prow[NMAX], pcol[NMAX];
-    __shared__ int piv[NMAX], pinv[NMAX], used[NMAX];
-    __shared__ int cand_i[16];
-    __shared__ double lad_sh;
-    const int h = blockIdx.x, n = a.n, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int bd = blockDim.x, nw = bd >> 6;
-    const long nn = (long)n * n;
-    double* aw = GLOBAL ? a.work + 8 + (size_t)a.M * nn + (size_t)h * nn : sm;
+struct GjView {
+    double* aw;
+    int* step;
+    int* piv;
+    double* dk;     /* |pivot| of step k */
+};
+
+__device__ __forceinline__ GjView gj_view(const MstepArgs& a, int h) {
+    const size_t nn = (size_t)a.n * a.n;
+    double* const tail = a.work + 8 + 2 * (size_t)a.M * nn;
+    GjView g;
+    g.aw = a.work + 8 + (size_t)a.M * nn + (size_t)h * nn;
+    g.step = (int*)tail + (size_t)h * 2 * a.n;
+    g.piv = g.step + a.n;
+    g.dk = tail + (size_t)a.M * a.n + (size_t)h * a.n;
+    return g;
+}
+
+/* 64-bit DPP move: both halves through the same lane pattern */
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ void argmax_step(double& bv, int& bi) {
+    const double ov = dpp_f64<CTRL>(bv);
+    const int oi = __builtin_amdgcn_update_dpp(0, bi, CTRL, 0xf, 0xf, false);
+    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+}
+
+/* (largest value, smallest index among equals) over the wave, uniform */
+__device__ __forceinline__ int wave_argmax(double bv, int bi) {
+    argmax_step<0xB1>(bv, bi);      /* quad_perm [1,0,3,2] */
+    argmax_step<0x4E>(bv, bi);      /* quad_perm [2,3,0,1] */
+    argmax_step<0x141>(bv, bi);     /* row_half_mirror: the other quad of 8 */
+    argmax_step<0x140>(bv, bi);     /* row_mirror: the other 8 of the row */
+    double best = readlane_f64(bv, 0);
+    int besti = __builtin_amdgcn_readlane(bi, 0);
+#pragma unroll
+    for (int row = 1; row < 4; ++row) {
+        const double ov = readlane_f64(bv, 16 * row);
+        const int oi = __builtin_amdgcn_readlane(bi, 16 * row);
+        if (ov > best || (ov == best && oi < besti)) { best = ov; besti = oi; }
+    }
+    return besti;
+}
+
+/* NB Gauss-Jordan steps on the panel v (columns k0 .., rows lane + 64 r),
+ * one wave.  rowbuf: NB doubles of LDS. */
+template <int R, int NB>
+__device__ __forceinline__ void gj_factor(const GjView& g, int n, int k0, double (&v)[R][NB],
+                                          double* rowbuf) {
+    const int lane = threadIdx.x;
+    const int nc = min(NB, n - k0);
+    int st[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        st[r] = i < n ? g.step[i] : n;          /* padding rows count as used */
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        if (c >= nc) continue;                      /* wave-uniform; keeps the loop unrollable */
+        const int k = k0 + c;
+        double bv = -1.0;
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            double av = fabs(v[r][c]);
+            if (av != av) av = __builtin_inf();     /* a NaN column still names an unused row */
+            if (st[r] < 0 && av > bv) { bv = av; bi = lane + 64 * r; }
+        }
+        const int p = wave_argmax(bv, bi);
+        const int pl = p & 63, pr = p >> 6;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (r == pr && lane == pl) {
+#pragma unroll
+                for (int c2 = 0; c2 < NB; ++c2) rowbuf[c2] = v[r][c2];
+            }
+        __syncthreads();
+        const double d = rowbuf[c];
+        /* 1/d: hardware estimate and two Newton steps (within an ulp) */
+        double inv = __builtin_amdgcn_rcp(d);
+        inv = fma(fma(-d, inv, 1.0), inv, inv);
+        inv = fma(fma(-d, inv, 1.0), inv, inv);
+        double prw[NB];
+#pragma unroll
+        for (int c2 = 0; c2 < NB; ++c2) prw[c2] = (c2 == c) ? inv : rowbuf[c2] * inv;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double pc = v[r][c];
+            const bool isp = lane + 64 * r == p;
+#pragma unroll
+            for (int c2 = 0; c2 < NB; ++c2) {
+                const double nv = ((c2 == c) ? 0.0 : v[r][c2]) - pc * prw[c2];
+                v[r][c2] = isp ? prw[c2] : nv;
+            }
+            if (isp) st[r] = k;
+        }
+        if (lane == 0) {
+            g.piv[k] = p;
+            g.dk[k] = fabs(d);
+        }
+        __syncthreads();                            /* rowbuf is rewritten next step */
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        if (i < n) g.step[i] = st[r];
+    }
+}
+
+template <int R, int NB>
+__device__ __forceinline__ void gj_store(const GjView& g, int n, int k0, const double (&v)[R][NB]) {
+#pragma unroll
+    for (int c2 = 0; c2 < NB; ++c2) {
+        if (k0 + c2 >= n) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = threadIdx.x + 64 * r;
+            if (i < n) g.aw[i + (size_t)(k0 + c2) * n] = v[r][c2];
+        }
+    }
+}
+
+/* A -> scratch, every row unused.  grid (x, M) */
+__global__ __launch_bounds__(256) void gj_init_kernel(MstepArgs a) {
+    const int h = blockIdx.y;
+    const GjView g = gj_view(a, h);
+    const long nn = (long)a.n * a.n;
     const double* A = a.A + (size_t)h * nn;
-    for (long q = tid; q < nn; q += bd) aw[q] = A[q];
-    for (int i = tid; i < n; i += bd) used[i] = 0;
-    if (tid == 0) lad_sh = 0.0;
-    __syncthreads();
-    /* lanes run over rows, waves over columns (no index division); the
-     * candidates of column c come from wave c % nw */
-    if (wv == 0) {
-        double cv = -1.0;
-        int ci = n;
-        for (int i = lane; i < n; i += 64) {
-            const double av = fabs(aw[i]);
-            if (av > cv) { cv = av; ci = i; }
-        }
-        for (int off = 32; off >= 1; off >>= 1) {
-            const double ov = __shfl_xor(cv, off, 64);
-            const int oi = __shfl_xor(ci, off, 64);
-            if (ov > cv || (ov == cv && oi < ci)) { cv = ov; ci = oi; }
-        }
-        if (lane == 0) cand_i[0] = ci;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (long)gridDim.x * blockDim.x)
+        g.aw[q] = A[q];
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < a.n; i += blockDim.x) g.step[i] = -1;
     }
-    __syncthreads();
-    for (int k = 0; k < n; ++k) {
-        const int p = cand_i[k % nw];
-        const double d = aw[p + (long)k * n];
-        const double inv = 1.0 / d;
-        if (tid < n) {
-            prow[tid] = (tid == k) ? inv : aw[p + (long)tid * n] * inv;
-        } else if (tid < 2 * n) {
-            const int i = tid - n;
-            pcol[i] = (i == p) ? 0.0 : aw[i + (long)k * n];
-        }
-        if (tid == 0) {
-            lad_sh += log(fabs(d));
-            piv[k] = p;
-            used[p] = 1;
-        }
-        __syncthreads();
-        for (int j = wv; j < n; j += nw) {
-            const double pj = prow[j];
-            double* col = aw + (long)j * n;
-            double cv = -1.0;
-            int ci = n;
-            for (int i = lane; i < n; i += 64) {
-                const double v = (i == p) ? pj : ((j == k) ? 0.0 : col[i]) - pcol[i] * pj;
-                col[i] = v;
-                if (j == k + 1 && !used[i]) {
-                    const double av = fabs(v);
-                    if (av > cv) { cv = av; ci = i; }
-                }
-            }
-            if (j == k + 1) {   /* wave-uniform: this wave holds the next pivot column */
-                for (int off = 32; off >= 1; off >>= 1) {
-                    const double ov = __shfl_xor(cv, off, 64);
-                    const int oi = __shfl_xor(ci, off, 64);
-                    if (ov > cv || (ov == cv && oi < ci)) { cv = ov; ci = oi; }
-                }
-                if (lane == 0) cand_i[j % nw] = ci;
-            }
-        }
-        __syncthreads();
+}
+
+/* the first panel: grid (M), one wave */
+template <int R, int NB>
+__global__ __launch_bounds__(64) void gj_panel_kernel(MstepArgs a) {
+    const GjView g = gj_view(a, blockIdx.x);
+    const int n = a.n;
+    __shared__ double rowbuf[NB];
+    double v[R][NB];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = threadIdx.x + 64 * r;
+#pragma unroll
+        for (int c2 = 0; c2 < NB; ++c2)
+            v[r][c2] = (i < n && c2 < n) ? g.aw[i + (size_t)c2 * n] : 0.0;
     }
-    for (int r = tid; r < n; r += bd) pinv[piv[r]] = r;
+    gj_factor<R, NB>(g, n, 0, v, rowbuf);
+    gj_store<R, NB>(g, n, 0, v);
+}
+
+/* Applies panel [kp, kp + NB) to every other column, one wave per column
+ * (grid (n + 1, M)).  With kn >= 0, block 0 instead updates the columns of
+ * the next panel [kn, kn + NB) in registers and factors them. */
+template <int R, int NB>
+__global__ __launch_bounds__(64) void gj_update_kernel(MstepArgs a, int kp, int kn) {
+    const int h = blockIdx.y, n = a.n, lane = threadIdx.x;
+    const GjView g = gj_view(a, h);
+    const int ncp = min(NB, n - kp);
+    __shared__ int pv[NB];
+    __shared__ double rowbuf[NB];
+    if (lane < ncp) pv[lane] = g.piv[kp + lane];
     __syncthreads();
-    double* W = a.W + (size_t)h * nn;
+    bool inP[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        bool f = false;
+        for (int q = 0; q < ncp; ++q) f |= (pv[q] == i);
+        inP[r] = f;
+    }
+    const double* pan = g.aw + (size_t)kp * n;      /* the applied panel's columns */
+    if (kn >= 0 && blockIdx.x == 0) {
+        const int ncn = min(NB, n - kn);
+        double v[R][NB];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + 64 * r;
+#pragma unroll
+            for (int c2 = 0; c2 < NB; ++c2)
+                v[r][c2] = (i < n && c2 < ncn && !inP[r]) ? g.aw[i + (size_t)(kn + c2) * n] : 0.0;
+        }
+        for (int q = 0; q < ncp; ++q) {
+            double pq[R], o[NB];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = lane + 64 * r;
+                pq[r] = i < n ? pan[i + (size_t)q * n] : 0.0;
+            }
+#pragma unroll
+            for (int c2 = 0; c2 < NB; ++c2)
+                o[c2] = c2 < ncn ? g.aw[pv[q] + (size_t)(kn + c2) * n] : 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int c2 = 0; c2 < NB; ++c2) v[r][c2] += pq[r] * o[c2];
+        }
+        gj_factor<R, NB>(g, n, kn, v, rowbuf);
+        gj_store<R, NB>(g, n, kn, v);
+        return;
+    }
+    const int j = (int)blockIdx.x - (kn >= 0 ? 1 : 0);
+    if (j < 0 || j >= n || (j >= kp && j < kp + NB) || (kn >= 0 && j >= kn && j < kn + NB)) return;
+    double* col = g.aw + (size_t)j * n;
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        acc[r] = (i < n && !inP[r]) ? col[i] : 0.0;
+    }
+    for (int q = 0; q < ncp; ++q) {
+        const double o = col[pv[q]];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + 64 * r;
+            if (i < n) acc[r] += pan[i + (size_t)q * n] * o;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        if (i < n) col[i] = acc[r];
+    }
+}
+
+/* W[r][m] = a[piv[r]][pinv[m]] and the log-determinant terms; grid (M),
+ * 1024 threads */
+__global__ __launch_bounds__(1024) void gj_gather_kernel(MstepArgs a) {
+    const int h = blockIdx.x, n = a.n, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nw = blockDim.x >> 6;
+    const GjView g = gj_view(a, h);
+    __shared__ int piv[kInvMax], pinv[kInvMax];
+    __shared__ double lsum[16];
+    double l = 0.0;
+    for (int r = tid; r < n; r += blockDim.x) {
+        piv[r] = g.piv[r];
+        l += log(g.dk[r]);
+    }
+    l = wave_sum_f64(l);
+    if (lane == 0) lsum[wv] = l;
+    __syncthreads();
+    for (int r = tid; r < n; r += blockDim.x) pinv[piv[r]] = r;
+    __syncthreads();
+    double* W = a.W + (size_t)h * n * n;
     for (int mcol = wv; mcol < n; mcol += nw) {
-        const double* src = aw + (long)pinv[mcol] * n;
-        for (int r = lane; r < n; r += 64) W[r + (long)mcol * n] = src[piv[r]];
+        const double* src = g.aw + (size_t)pinv[mcol] * n;
+        for (int r = lane; r < n; r += 64) W[r + (size_t)mcol * n] = src[piv[r]];
     }
-    if (tid == 0) set_ldet(a, h, lad_sh);
+    if (tid == 0) {
+        double lad = 0.0;                           /* log|det A| = sum_k log|d_k| */
+        for (int w = 0; w < nw; ++w) lad += lsum[w];
+        set_ldet(a, h, lad);
+    }
+}
+
+template <int R, int NB>
+static void launch_gj(const MstepArgs& ma, hipStream_t st) {
+    const int n = ma.n;
+    const long nn = (long)n * n;
+    const unsigned gi = (unsigned)std::min<long>((nn + 255) / 256, 64);
+    hipLaunchKernelGGL(gj_init_kernel, dim3(gi, ma.M), dim3(256), 0, st, ma);
+    hipLaunchKernelGGL((gj_panel_kernel<R, NB>), dim3(ma.M), dim3(64), 0, st, ma);
+    const int npan = (n + NB - 1) / NB;
+    for (int b = 1; b < npan; ++b)
+        hipLaunchKernelGGL((gj_update_kernel<R, NB>), dim3(n + 1, ma.M), dim3(64), 0, st, ma,
+                           (b - 1) * NB, b * NB);
+    if (npan > 1)
+        hipLaunchKernelGGL((gj_update_kernel<R, NB>), dim3(n, ma.M), dim3(64), 0, st, ma,
+                           (npan - 1) * NB, -1);
+    hipLaunchKernelGGL(gj_gather_kernel, dim3(ma.M), dim3(1024), 0, st, ma);
 }
 
 }  // namespace
@@ -553,21 +776,22 @@ int launch_mstep(const MstepArgs& ma, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-static size_t inverse_lds_bytes(int n) { return (size_t)n * n * sizeof(double); }
+int mstep_configure(int n) { return (n >= 1 && n <= kInvMax) ? 0 : -1; }
 
-int mstep_configure(int n) {
-    if (n > kLdsInvMax) return 0;
-    return hipFuncSetAttribute((const void*)inverse_gj_kernel<false>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)inverse_lds_bytes(n)) == hipSuccess ? 0 : -1;
+/* scratch doubles of the inverse after the first 8 + M n n of work */
+size_t inverse_work_doubles(int n, int M) {
+    return (size_t)M * n * n + 2 * (size_t)M * n;
 }
 
 int launch_prepare(const MstepArgs& ma, hipStream_t st) {
-    if (ma.n > 1024) return -1;
-    if (ma.n <= kLdsInvMax)
-        hipLaunchKernelGGL(inverse_gj_kernel<false>, dim3(ma.M), dim3(1024), inverse_lds_bytes(ma.n), st, ma);
-    else
-        hipLaunchKernelGGL(inverse_gj_kernel<true>, dim3(ma.M), dim3(1024), 0, st, ma);
+    /* registers per lane of a panel: R rows x NB columns <= 64 doubles */
+    const int n = ma.n;
+    if (n < 1 || n > kInvMax) return -1;
+    if (n <= 64) launch_gj<1, 32>(ma, st);
+    else if (n <= 128) launch_gj<2, 16>(ma, st);
+    else if (n <= 256) launch_gj<4, 16>(ma, st);
+    else if (n <= 512) launch_gj<8, 8>(ma, st);
+    else launch_gj<16, 4>(ma, st);
     const long total = (long)ma.n * ma.n + ma.n + (long)ma.n * ma.m;
     const unsigned gx = (unsigned)std::min<long>((total + 255) / 256, 256);
     hipLaunchKernelGGL(prepare_tail_kernel, dim3(gx, ma.M), dim3(256), 0, st, ma);
