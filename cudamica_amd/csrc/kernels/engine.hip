@@ -292,17 +292,18 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
         return 0;
     };
 #define ALLOC(ptr, bytes) do { int _r = alloc((void**)&(ptr), (bytes)); if (_r) { amica_engine_destroy(e); return _r; } } while (0)
-    ALLOC(e->xw, (size_t)T * NP * sizeof(float));
+    const size_t Tp = (size_t)T + kEstepRowPad;   /* rows the E-step may read */
+    ALLOC(e->xw, Tp * NP * sizeof(float));
     if (e->fp8) {
         ALLOC(e->xq, (size_t)T * NP);
         ALLOC(e->xscale, (size_t)NP * sizeof(float));
-        ALLOC(e->b, (size_t)T * e->ldb * sizeof(float));
+        ALLOC(e->b, Tp * e->ldb * sizeof(float));
     } else if (e->bf16) {
         ALLOC(e->xbf, 2 * (size_t)T * NP * sizeof(uint16_t));   /* hi, then lo */
-        ALLOC(e->b, (size_t)T * e->ldb * sizeof(float));
+        ALLOC(e->b, Tp * e->ldb * sizeof(float));
     } else {
         ALLOC(e->xw64, (size_t)T * NP * sizeof(double));
-        ALLOC(e->b64, (size_t)T * e->ldb * sizeof(double));
+        ALLOC(e->b64, Tp * e->ldb * sizeof(double));
     }
     /* BF16: g and x packed [T32/8][cols][8] for the statistic GEMM; the
      * padding samples up to T32 stay zero */

@@ -71,16 +71,14 @@ __device__ __forceinline__ float dpp(float v) {
                                                                  CTRL, ROWS, BANKS, BOUND));
 }
 
-/* Sum over the 64 lanes; the total lands in lane 63 (other lanes hold
- * partial sums).  Row shifts 1, 2 (groups of 4), 4, 8 (row of 16, in lane
- * 15 of each row), then row broadcasts 15 and 31 across rows. */
-__device__ __forceinline__ float wave_sum_lane63(float v) {
+/* Sum over each row of 16 lanes; the row total lands in the row's lane 15
+ * (the other lanes hold partial sums).  Row shifts by 1, 2, 4 and 8 with
+ * zeros shifted in, each one v_add_f32 with a DPP source. */
+__device__ __forceinline__ float row_sum_lane15(float v) {
     v += dpp<0x111, 0xf, 0xf, true>(v);
     v += dpp<0x112, 0xf, 0xf, true>(v);
-    v += dpp<0x114, 0xf, 0xe, true>(v);
-    v += dpp<0x118, 0xf, 0xc, true>(v);
-    v += dpp<0x142, 0xa, 0xf, false>(v);
-    v += dpp<0x143, 0xc, 0xf, false>(v);
+    v += dpp<0x114, 0xf, 0xf, true>(v);
+    v += dpp<0x118, 0xf, 0xf, true>(v);
     return v;
 }
 
@@ -101,8 +99,9 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     const int i = cb * 64 + lane;
     const bool act = i < a.n;
 
-    /* per-wave channel sums of one tile, double buffered: [buf][wave][sample] */
-    __shared__ __align__(16) float Lp[2][kMaxEstepWaves][kS];
+    /* per-row channel sums of one tile, double buffered:
+     * [buf][wave][row of 16 lanes][sample] */
+    __shared__ __align__(16) float Lp[2][kMaxEstepWaves][4][kS];
     __shared__ float dc2[AMICA_MAX_M];
     /* per-lane FP32 level-2 totals, [blockDim.x][NSP] (odd stride: the
      * lanes of a wave hit distinct banks) */
@@ -168,9 +167,8 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 #pragma unroll
     for (int s = 0; s < kS; ++s) {
         const long t = t0 + s;
-        const bool ok = act && t < t1;
-        if constexpr (B64) bn64[s] = ok ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
-        else bn[s] = ok ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
+        if constexpr (B64) bn64[s] = act ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
+        else bn[s] = act ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
     }
 
     for (long tb = t0; tb < t1; tb += kS) {
@@ -183,15 +181,16 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             else BV[s] = bn[s];
         }
         /* issue the next tile's b loads and this tile's x loads now; they
-         * complete under this tile's math */
+         * complete under this tile's math.  Rows past t1 are read without a
+         * check (the next range's samples, or the kEstepRowPad zero rows
+         * after the last one) and get posterior 0. */
 #pragma unroll
         for (int s = 0; s < kS; ++s) {
             const long tn = tb + kS + s;
-            const bool okn = act && tn < t1;
-            if constexpr (B64) bn64[s] = okn ? a.b64[(size_t)tn * a.ldb + bcol] : 0.0;
-            else bn[s] = okn ? a.b[(size_t)tn * a.ldb + bcol] : 0.f;
+            if constexpr (B64) bn64[s] = act ? a.b64[(size_t)tn * a.ldb + bcol] : 0.0;
+            else bn[s] = act ? a.b[(size_t)tn * a.ldb + bcol] : 0.f;
             const long t = tb + s;
-            XV[s] = (act && t < t1) ? a.x[(size_t)t * a.NP + i] : 0.f;
+            XV[s] = act ? a.x[(size_t)t * a.NP + i] : 0.f;
         }
         /* ---- phase 1: densities, component posteriors, channel sums ---- */
         float QS[kS];
@@ -235,10 +234,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             QS[s] = act ? off + fast_log2(se) : 0.f;
         }
 #pragma unroll
-        for (int s = 0; s < kS; ++s) QS[s] = wave_sum_lane63(QS[s]);
-        if (lane == 63) {
+        for (int s = 0; s < kS; ++s) QS[s] = row_sum_lane15(QS[s]);
+        if ((lane & 15) == 15) {
 #pragma unroll
-            for (int s = 0; s < kS; ++s) Lp[buf][w][s] = QS[s];
+            for (int s = 0; s < kS; ++s) Lp[buf][w][lane >> 4][s] = QS[s];
         }
         __syncthreads();
         /* ---- model posteriors: lane hh*kS + s holds model hh, sample s ---- */
@@ -251,7 +250,9 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             float l = -3.0e38f;
             if (hh < a.M) {
                 l = dc2[hh];
-                for (int c = 0; c < a.CW; ++c) l += Lp[buf][hh * a.CW + c][s];
+                for (int c = 0; c < a.CW; ++c)
+#pragma unroll
+                    for (int row = 0; row < 4; ++row) l += Lp[buf][hh * a.CW + c][row][s];
             }
             /* logsumexp over the models: the lanes of one sample are kS = 2
              * apart inside the first row of 16 lanes (M <= 8), so row

@@ -455,11 +455,23 @@ static int launch_unmix_reg(const void* xa, const uint16_t* xlo, const uint16_t*
     while (nwaves % waves) --waves;
     const int groups = nwaves / waves;
     const long ntiles = (T + 15) / 16;
-    const long gx = std::max(1L, std::min<long>(ntiles, (long)cus * std::max(1, 8 / waves) / groups));
-    const dim3 g((unsigned)gx, groups), blk(64 * waves);
+    const dim3 blk(64 * waves);
+    /* as many workgroups as the CUs hold at once (the occupancy of each
+     * instantiation, queried once per block size) */
 #define AMICA_UNMIX_REG(C, K)                                                                   \
-    hipLaunchKernelGGL((unmix_reg_kernel<C, K, Q8>), g, blk, 0, st, xa, xlo, Whi, Wlo, wc, b,  \
-                       T, ldb, ntiles)
+    do {                                                                                        \
+        static int bpc[9] = {0};                                                                \
+        if (bpc[waves] == 0) {                                                                  \
+            int nb = 0;                                                                         \
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                  \
+                    &nb, unmix_reg_kernel<C, K, Q8>, (int)blk.x, 0) != hipSuccess || nb < 1)   \
+                nb = 1;                                                                         \
+            bpc[waves] = nb;                                                                    \
+        }                                                                                       \
+        const long gx = std::max(1L, std::min<long>(ntiles, (long)cus * bpc[waves] / groups));  \
+        hipLaunchKernelGGL((unmix_reg_kernel<C, K, Q8>), dim3((unsigned)gx, groups), blk, 0,   \
+                           st, xa, xlo, Whi, Wlo, wc, b, T, ldb, ntiles);                      \
+    } while (0)
     if (CT == 4 && KS == 2) AMICA_UNMIX_REG(4, 2);
     else if (CT == 4 && KS == 4) AMICA_UNMIX_REG(4, 4);
     else if (CT == 2 && KS == 6) AMICA_UNMIX_REG(2, 6);

@@ -7,8 +7,10 @@ relative, identical branch sequences (Newton / natural gradient, LL
 decrease) and the final W within an Amari distance of 1e-4, wherever the
 trajectory itself allows it: a full run is held to max(1e-6, 4x the
 oracle's own spread when its input is rounded to float32 or carries
-float32-sized noise), iteration by iteration.  BF16 path:
-final LL within 1e-4 relative of the FP64 oracle.
+float32-sized noise), iteration by iteration.  BF16 path: E-step
+statistics within 1e-3 and the first LL within 1e-6; later iterations
+within max(1e-6, 4x the oracle's spread under input perturbations the size
+of the BF16 unmixing's rounding, 2^-17).
 """
 import numpy as np
 import pytest
@@ -232,13 +234,23 @@ def test_full_run_fp32_newton(built):
 
 
 def test_full_run_bf16_models(built):
+    """BF16 GEMMs round W x to ~2^-17 and g, x to 2^-9 in Phi.  This
+    trajectory is ill-conditioned (the oracle itself moves by 3e-5 in LL
+    within 4 iterations under float32-sized input perturbations, 6e-5 under
+    2^-17-sized ones), so the bar per iteration is max(1e-6, 4x the oracle's
+    spread under 2^-17 perturbations of its input), and the first
+    iteration (the E-step alone) must agree within 1e-6."""
     w = ca.WORKLOADS["n128_T2M_M4"]
     T = 100000
     _say(f"[run bf16] n128 M4 subsample T={T}, 8 iterations")
     fin, logs, gst, glog = _run_both(w, T, 8, ca.PREC_BF16)
     rel = np.abs(glog["ll"] - logs["ll"]) / np.abs(logs["ll"])
-    _say(f"  bf16 LL rel err per iteration: {np.array2string(rel, precision=2)}")
-    assert rel[-1] <= 1e-4
+    env, _ = _oracle_envelope(w, T, 8, w.newt_start_iter, fin, logs, eps=2.0 ** -17)
+    fmt = dict(formatter={"float_kind": lambda v: f"{v:.2e}"})
+    _say(f"  bf16 LL rel err per iteration: {np.array2string(rel, **fmt)}")
+    _say(f"  oracle spread under 2^-17 input perturbation: {np.array2string(env, **fmt)}")
+    assert rel[0] <= 1e-6
+    assert np.all(rel <= np.maximum(1e-6, 4.0 * env))
 
 
 def test_graph_replay_matches_eager(built):
