@@ -186,16 +186,20 @@ __device__ __forceinline__ i16x8 fp8x8_to_bf16x8(uint2 q) {
  *     (~2^-16 relative);
  *   Q8 = true: x stored as FP8 e4m3 q with the per-channel scales folded
  *     into W (W~ = split(W diag(s))), b = q W_hi + q W_lo, q exact in bf16.
- * A workgroup of waves covers one column group (blockIdx.y); wave w owns
- * 16 CT columns and keeps their W_hi / W_lo B-fragments for all NP = 32 KS
- * inputs in registers for the whole launch.  The waves of a workgroup walk
- * the same 16-sample tiles (tile = blockIdx.x, += gridDim.x), so each x tile
- * comes from HBM once per column group and from L1/L2 for the other waves;
- * the next tile's A fragments load while the current tile's MFMAs run.
- * HBM traffic per launch: x (4 NP or NP bytes per sample, per column group)
- * + b (4 ldb bytes per sample). */
+ * A workgroup of 8 waves covers one group of 128 CT columns (blockIdx.y);
+ * wave w owns 16 CT of them and keeps their W_hi / W_lo fragments for all
+ * NP = 32 KS inputs in registers for the whole launch (waves past ldb only
+ * load).  The workgroup walks 16-sample tiles (tile = blockIdx.x, +=
+ * gridDim.x): every tile is read from HBM once per column group, 16 bytes
+ * per lane, kUnmixStages tiles ahead of its use, written to one of two LDS
+ * slots (rows padded by 16 bytes, so that the 16 lanes reading one
+ * fragment column hit distinct banks) and read from there by every wave.
+ * One barrier per tile.  HBM traffic per launch: x (4 NP or NP bytes per
+ * sample, per column group) + b (4 ldb bytes per sample). */
+constexpr int kUnmixStages = 4;
+
 template <int CT, int KS, bool Q8>
-__global__ __launch_bounds__(512) void unmix_reg_kernel(const void* __restrict__ xa,
+__global__ __launch_bounds__(512) void unmix_lds_kernel(const void* __restrict__ xa,
                                                          const uint16_t* __restrict__ xlo,
                                                          const uint16_t* __restrict__ Whi,
                                                          const uint16_t* __restrict__ Wlo,
@@ -203,68 +207,95 @@ __global__ __launch_bounds__(512) void unmix_reg_kernel(const void* __restrict__
                                                          float* __restrict__ b, long T, int ldb,
                                                          long ntiles) {
     constexpr int NP = 32 * KS;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int c0 = (blockIdx.y * (blockDim.x >> 6) + w) * 16 * CT;
+    constexpr int PARTS = Q8 ? 1 : 2;              /* fp8 q, or bf16 hi and lo */
+    constexpr int ROWB = Q8 ? NP : 2 * NP;         /* bytes of one sample row of one part */
+    constexpr int RS = ROWB + 16;                  /* LDS row stride */
+    constexpr int SLOT = PARTS * 16 * RS;          /* LDS bytes of one tile */
+    constexpr int CPR = ROWB / 16;                 /* 16-byte chunks per row */
+    constexpr int CHUNKS = PARTS * 16 * CPR;       /* chunks per tile */
+    constexpr int NCH = (CHUNKS + 511) / 512;      /* chunks per lane */
+    constexpr int D = kUnmixStages;
+    __shared__ __align__(16) unsigned char lds[2 * SLOT];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int c0 = (blockIdx.y * 8 + w) * 16 * CT;
+    const bool computes = c0 < ldb;                /* wave-uniform */
     const int kofs = (lane >> 4) * 8;
     i16x8 bhi[CT][KS], blo[CT][KS];
     float bias[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
         const int col = c0 + c * 16 + (lane & 15);
-        bias[c] = wc[col];
+        bias[c] = computes ? wc[col] : 0.f;
 #pragma unroll
         for (int k = 0; k < KS; ++k) {
             const size_t wofs = (size_t)col * NP + k * 32 + kofs;
-            bhi[c][k] = *(const i16x8*)(Whi + wofs);
-            blo[c][k] = *(const i16x8*)(Wlo + wofs);
+            bhi[c][k] = computes ? *(const i16x8*)(Whi + wofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
+            blo[c][k] = computes ? *(const i16x8*)(Wlo + wofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
         }
     }
-    constexpr int NA = Q8 ? 1 : 2;          /* A operands per k-step */
-    i16x8 an[NA][KS];                        /* bf16 path: loaded A; FP8 path unused */
-    uint2 qn[KS];                            /* FP8 path: loaded bytes */
-    auto load_a = [&](long tile) {
-        const long t = tile * 16 + (lane & 15);
-        const bool ok = tile < ntiles && t < T;
+    const long first = blockIdx.x, stride = gridDim.x;
+    const long ntl = first < ntiles ? (ntiles - 1 - first) / stride + 1 : 0;   /* block-uniform */
+
+    auto gload = [&](long j, uint4 (&r)[NCH]) {
+        const long tile = first + j * stride;
 #pragma unroll
-        for (int k = 0; k < KS; ++k) {
-            const size_t xofs = (size_t)t * NP + k * 32 + kofs;
-            if constexpr (Q8) {
-                qn[k] = ok ? *(const uint2*)((const uint8_t*)xa + xofs) : make_uint2(0u, 0u);
-            } else {
-                an[0][k] = ok ? *(const i16x8*)((const uint16_t*)xa + xofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
-                an[1][k] = ok ? *(const i16x8*)(xlo + xofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < NCH; ++q) {
+            const int c = tid + q * 512;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (c < CHUNKS && j < ntl) {
+                const int part = c / (16 * CPR);
+                const int rem = c - part * 16 * CPR;
+                const int srow = rem / CPR, cc = rem - srow * CPR;
+                const long t = tile * 16 + srow;
+                if (t < T) {
+                    const unsigned char* src =
+                        Q8 ? (const unsigned char*)xa + (size_t)t * NP + cc * 16
+                           : (const unsigned char*)(part ? xlo : (const uint16_t*)xa) + (size_t)t * NP * 2 + cc * 16;
+                    v = *(const uint4*)src;
+                }
+            }
+            r[q] = v;
+        }
+    };
+    auto lwrite = [&](int slot, const uint4 (&r)[NCH]) {
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+            const int c = tid + q * 512;
+            if (c < CHUNKS) {
+                const int part = c / (16 * CPR);
+                const int rem = c - part * 16 * CPR;
+                const int srow = rem / CPR, cc = rem - srow * CPR;
+                *(uint4*)(lds + slot * SLOT + part * 16 * RS + srow * RS + cc * 16) = r[q];
             }
         }
     };
-    long tile = blockIdx.x;
-    load_a(tile);
-    for (; tile < ntiles; tile += gridDim.x) {
-        i16x8 ahi[KS], alo[KS];
-#pragma unroll
-        for (int k = 0; k < KS; ++k) {
-            if constexpr (Q8) {
-                ahi[k] = fp8x8_to_bf16x8(qn[k]);
-            } else {
-                ahi[k] = an[0][k];
-                alo[k] = an[1][k];
-            }
-        }
-        load_a(tile + gridDim.x);
+    auto compute = [&](int slot, long j) {
+        const long tile = first + j * stride;
+        const unsigned char* base = lds + slot * SLOT + (lane & 15) * RS;
         f32x4 acc[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) acc[c] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int k = 0; k < KS; ++k)
+        for (int k = 0; k < KS; ++k) {
+            i16x8 ahi, alo;
+            if constexpr (Q8) {
+                ahi = fp8x8_to_bf16x8(*(const uint2*)(base + k * 32 + kofs));
+            } else {
+                ahi = *(const i16x8*)(base + (k * 32 + kofs) * 2);
+                alo = *(const i16x8*)(base + 16 * RS + (k * 32 + kofs) * 2);
+            }
 #pragma unroll
             for (int c = 0; c < CT; ++c) {
                 if constexpr (!Q8)
-                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, alo[k]),
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, alo),
                                                                      __builtin_bit_cast(bf16x8, bhi[c][k]), acc[c], 0, 0, 0);
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ahi[k]),
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ahi),
                                                                  __builtin_bit_cast(bf16x8, blo[c][k]), acc[c], 0, 0, 0);
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ahi[k]),
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ahi),
                                                                  __builtin_bit_cast(bf16x8, bhi[c][k]), acc[c], 0, 0, 0);
             }
+        }
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             const int col = c0 + c * 16 + (lane & 15);
@@ -272,6 +303,23 @@ __global__ __launch_bounds__(512) void unmix_reg_kernel(const void* __restrict__
             for (int r = 0; r < 4; ++r) {
                 const long t = tile * 16 + (lane >> 4) * 4 + r;
                 if (t < T) b[(size_t)t * ldb + col] = acc[c][r] - bias[c];
+            }
+        }
+    };
+
+    uint4 R[D][NCH];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gload(d, R[d]);
+    /* unrolled by D so that stage u always lives in R[u] */
+    for (long j0 = 0; j0 < ntl; j0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            const long j = j0 + u;
+            if (j < ntl) {                         /* block-uniform */
+                lwrite((int)(j & 1), R[u]);        /* tile j, loaded D tiles ago */
+                gload(j + D, R[u]);
+                __syncthreads();                   /* slot j&1 written; slot (j-1)&1 free */
+                if (computes) compute((int)(j & 1), j);
             }
         }
     }
@@ -429,15 +477,15 @@ int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-/* Column tiles per wave and k-steps of the register-resident kernels, or
- * CT = 0 where none fits (B fragments: 4 CT KS VGPRs per bf16 part). */
-static void unmix_shape(int NP, bool q8, int& CT, int& KS) {
+/* Column tiles per wave and k-steps of the register-resident kernel, or
+ * CT = 0 where none fits (W fragments: 8 CT KS VGPRs). */
+static void unmix_shape(int NP, int& CT, int& KS) {
     KS = NP / 32;
     switch (NP) {
         case 64: CT = 4; break;
         case 128: CT = 4; break;
         case 192: CT = 2; break;
-        case 256: CT = q8 ? 2 : 1; break;
+        case 256: CT = 2; break;
         default: CT = 0; break;
     }
 }
@@ -447,36 +495,29 @@ static int launch_unmix_reg(const void* xa, const uint16_t* xlo, const uint16_t*
                             const uint16_t* Wlo, const float* wc, float* b, long T, int NP, int ldb,
                             int cus, hipStream_t st) {
     int CT, KS;
-    unmix_shape(NP, Q8, CT, KS);
+    unmix_shape(NP, CT, KS);
     if (CT == 0 || ldb % (16 * CT)) return -1;
-    /* waves per workgroup: the largest divisor of the wave count <= 8 */
-    const int nwaves = ldb / (16 * CT);
-    int waves = 8;
-    while (nwaves % waves) --waves;
-    const int groups = nwaves / waves;
+    const int groups = (ldb + 128 * CT - 1) / (128 * CT);
     const long ntiles = (T + 15) / 16;
-    const dim3 blk(64 * waves);
-    /* as many workgroups as the CUs hold at once (the occupancy of each
-     * instantiation, queried once per block size) */
+    /* as many workgroups as the CUs hold at once (queried once per kernel) */
 #define AMICA_UNMIX_REG(C, K)                                                                   \
     do {                                                                                        \
-        static int bpc[9] = {0};                                                                \
-        if (bpc[waves] == 0) {                                                                  \
+        static int bpc = 0;                                                                     \
+        if (bpc == 0) {                                                                         \
             int nb = 0;                                                                         \
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                  \
-                    &nb, unmix_reg_kernel<C, K, Q8>, (int)blk.x, 0) != hipSuccess || nb < 1)   \
+                    &nb, unmix_lds_kernel<C, K, Q8>, 512, 0) != hipSuccess || nb < 1)          \
                 nb = 1;                                                                         \
-            bpc[waves] = nb;                                                                    \
+            bpc = nb;                                                                           \
         }                                                                                       \
-        const long gx = std::max(1L, std::min<long>(ntiles, (long)cus * bpc[waves] / groups));  \
-        hipLaunchKernelGGL((unmix_reg_kernel<C, K, Q8>), dim3((unsigned)gx, groups), blk, 0,   \
-                           st, xa, xlo, Whi, Wlo, wc, b, T, ldb, ntiles);                      \
+        const long gx = std::max(1L, std::min<long>(ntiles, (long)cus * bpc / groups));         \
+        hipLaunchKernelGGL((unmix_lds_kernel<C, K, Q8>), dim3((unsigned)gx, groups), dim3(512), \
+                           0, st, xa, xlo, Whi, Wlo, wc, b, T, ldb, ntiles);                   \
     } while (0)
     if (CT == 4 && KS == 2) AMICA_UNMIX_REG(4, 2);
     else if (CT == 4 && KS == 4) AMICA_UNMIX_REG(4, 4);
     else if (CT == 2 && KS == 6) AMICA_UNMIX_REG(2, 6);
     else if (CT == 2 && KS == 8) AMICA_UNMIX_REG(2, 8);
-    else if (CT == 1 && KS == 8) AMICA_UNMIX_REG(1, 8);
     else return -1;
 #undef AMICA_UNMIX_REG
     return hipGetLastError() == hipSuccess ? 0 : -1;
