@@ -57,8 +57,7 @@ namespace amica {
 
 namespace {
 
-constexpr int kS = 2;          /* samples per barrier tile */
-constexpr int kFlush = 32 / kS; /* tiles summed in FP32 registers (32 samples) */
+constexpr int kS = 2;          /* samples per barrier tile (M = 1 kernel) */
 constexpr int kFlush2 = 32;    /* register flushes summed in FP32 LDS */
 
 __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
@@ -90,6 +89,10 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
 
 template <int MJ, bool B64>
 __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
+    /* samples per barrier tile: 2, or 1 where m >= 4 so that the per-sample
+     * values stay within 128 VGPRs (two workgroups per CU) */
+    constexpr int kS = MJ >= 4 ? 1 : 2;
+    constexpr int kFlush = 32 / kS;     /* tiles summed in FP32 registers (32 samples) */
     constexpr int NSJ = kStatsPerJ;
     constexpr int NS = NSJ * MJ + kStatsPerI;
     const int lane = threadIdx.x & 63;
@@ -158,6 +161,8 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
      * current 8-sample group (2 samples each) shift through gw0..gw3. */
     uint32_t gw0 = 0, gw1 = 0, gw2 = 0, gw3 = 0;
     int gq = 0;
+    float gpend = 0.f;       /* one sample per tile: the first sample of a dword */
+    int godd = 0;
     long gbase = t0;
     uint4* const g8 = (uint4*)a.g;
 
@@ -254,15 +259,18 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 #pragma unroll
                     for (int row = 0; row < 4; ++row) l += Lp[buf][hh * a.CW + c][row][s];
             }
-            /* logsumexp over the models: the lanes of one sample are kS = 2
+            /* logsumexp over the models: the lanes of one sample are kS
              * apart inside the first row of 16 lanes (M <= 8), so row
-             * rotations by 2, 4 and 8 reach all of them */
-            static_assert(kS == 2, "the model reduction assumes 2 samples per tile");
-            float lmax = fmaxf(l, dpp<0x122, 0xf, 0xf, false>(l));      /* row_ror:2 */
+             * rotations by (1,) 2, 4 and 8 reach all of them */
+            float lmax = l;
+            if constexpr (kS == 1) lmax = fmaxf(lmax, dpp<0x121, 0xf, 0xf, false>(lmax));  /* row_ror:1 */
+            lmax = fmaxf(lmax, dpp<0x122, 0xf, 0xf, false>(lmax));      /* row_ror:2 */
             lmax = fmaxf(lmax, dpp<0x124, 0xf, 0xf, false>(lmax));      /* row_ror:4 */
             lmax = fmaxf(lmax, dpp<0x128, 0xf, 0xf, false>(lmax));      /* row_ror:8 */
             const float e = (hh < a.M) ? fast_exp2(l - lmax) : 0.f;
-            float esum = e + dpp<0x122, 0xf, 0xf, false>(e);
+            float esum = e;
+            if constexpr (kS == 1) esum += dpp<0x121, 0xf, 0xf, false>(esum);
+            esum += dpp<0x122, 0xf, 0xf, false>(esum);
             esum += dpp<0x124, 0xf, 0xf, false>(esum);
             esum += dpp<0x128, 0xf, 0xf, false>(esum);
             const float P2 = lmax + fast_log2(esum);
@@ -305,12 +313,22 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             if (!a.g_bf16 && t < t1) ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
         }
         if (a.g_bf16) {
-            static_assert(kS == 2, "one packed dword per tile");
-            gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = pack_bf16x2(GV[0], GV[1]);
-            if (++gq == 4) {
-                g8[(size_t)(gbase >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
-                gq = 0;
-                gbase = tb + kS;
+            bool word = true;                      /* a dword (2 samples) complete */
+            float g0, g1;
+            if constexpr (kS == 2) {
+                g0 = GV[0]; g1 = GV[1];
+            } else {
+                g0 = gpend; g1 = GV[0];
+                gpend = GV[0];
+                word = (godd ^= 1) == 0;
+            }
+            if (word) {
+                gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = pack_bf16x2(g0, g1);
+                if (++gq == 4) {
+                    g8[(size_t)(gbase >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
+                    gq = 0;
+                    gbase = tb + kS;
+                }
             }
         }
         buf ^= 1;
@@ -333,7 +351,8 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             }
         }
     }
-    if (a.g_bf16 && gq != 0) {   /* a partial group at the end of the recording */
+    if (a.g_bf16 && (gq != 0 || godd != 0)) {   /* a partial group at the end of the recording */
+        if (godd) { gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = pack_bf16x2(gpend, 0.f); ++gq; }
         while (gq < 4) { gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = 0u; ++gq; }
         g8[(size_t)(gbase >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
     }
