@@ -112,7 +112,7 @@ def measured_traffic(workload, prec):
     """HBM bytes per E-step launch measured with rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE passes (scaled by the copy probe), kept in profiles/ as JSON."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"traffic_{workload}_{'bf16' if prec == 16 else 'fp32'}.json")
+                        f"traffic_{workload}_{PREC_NAME[prec]}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:

@@ -97,6 +97,7 @@ struct amica_engine {
     int nchunks = 0;
     double* gxpart = nullptr;
     double* raw = nullptr;
+    double* sumtmp = nullptr;      /* first-pass group sums of the partials */
     double* gxtot = nullptr;
     /* parameters and work */
     double *A = nullptr, *c = nullptr, *alpha = nullptr, *mu = nullptr, *beta = nullptr,
@@ -207,7 +208,7 @@ struct amica_engine {
         else
             LCHK(launch_gx_f32((const float*)g, xw, gxpart, o.T, NP, ldb, nchunks, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[3], st));
-        LCHK(launch_reduce(part, nblocks, pstride, gxpart, nchunks, raw, gxtot, ma, st));
+        LCHK(launch_reduce(part, nblocks, pstride, gxpart, nchunks, raw, gxtot, sumtmp, ma, st));
         return 0;
     }
 
@@ -344,6 +345,7 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     ALLOC(e->part, (size_t)e->nblocks_cap * e->pstride * sizeof(double));
     ALLOC(e->gxpart, (size_t)e->nchunks_cap * e->ldb * NP * sizeof(double));
     ALLOC(e->raw, e->pstride * sizeof(double));
+    ALLOC(e->sumtmp, (size_t)kSumGroups * std::max(e->pstride, (size_t)e->ldb * NP) * sizeof(double));
     ALLOC(e->gxtot, (size_t)e->ldb * NP * sizeof(double));
     const size_t nn = (size_t)n * n, nm = (size_t)n * m;
     ALLOC(e->A, M * nn * sizeof(double));
@@ -401,7 +403,7 @@ int amica_engine_destroy(amica_engine* e) {
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     void* ptrs[] = {e->xraw, e->xw, e->xw64, e->xbf, e->x8, e->xq, e->xscale, e->b, e->b64, e->g, e->part, e->gxpart,
-                    e->raw, e->gxtot, e->A, e->c, e->alpha, e->mu, e->beta, e->rho, e->gm, e->W,
+                    e->raw, e->sumtmp, e->gxtot, e->A, e->c, e->alpha, e->mu, e->beta, e->rho, e->gm, e->W,
                     e->ldet, e->lconst, e->Wg64, e->wc64, e->Whi, e->Wlo, e->wc, e->cst,
                     e->cst64, e->stats, e->sigma2, e->cnew, e->phis,
                     e->kappa, e->lambda, e->Bm, e->work, e->lu, e->bflag, e->newton, e->ds,

@@ -149,8 +149,12 @@ struct MstepArgs {
 
 /* Sum E-step partials and the statistic-GEMM partials, then write the
  * canonical statistics (including phi through the unmixing identity). */
+/* Partials are summed in up to kSumGroups groups first: tmp holds
+ * kSumGroups * max(pstride, ldb NP) doubles. */
+constexpr int kSumGroups = 64;
 int launch_reduce(const double* part, int nparts, size_t pstride, const double* gxpart,
-                  int nchunks, double* raw, double* gxtot, const MstepArgs& ma, hipStream_t st);
+                  int nchunks, double* raw, double* gxtot, double* tmp, const MstepArgs& ma,
+                  hipStream_t st);
 int launch_mstep(const MstepArgs& ma, hipStream_t st);
 /* W = A^-1, ldet, GEMM-precision W, W c, E-step constants. */
 int launch_prepare(const MstepArgs& ma, hipStream_t st);

@@ -60,26 +60,32 @@ __device__ double block_sum(double v, double* sh) {
 /* reductions                                                                 */
 /* ------------------------------------------------------------------------ */
 
-/* out[e] = sum_p part[p][e].  A workgroup covers 64 elements (lanes); its
- * kSumWaves waves take the partials p = w, w + kSumWaves, ... and their
- * totals are added in wave order, so the result does not depend on timing. */
+/* out[y][e] = sum of part[p][e] over the partials p of group y = blockIdx.y
+ * ([y PG, (y + 1) PG)).  A workgroup covers 64 elements (lanes); its
+ * kSumWaves waves take the group's partials p = w, w + kSumWaves, ... and
+ * their totals are added in wave order, so the result does not depend on
+ * timing. */
 constexpr int kSumWaves = 16;
 
 __global__ __launch_bounds__(64 * kSumWaves) void sum_parts_kernel(const double* __restrict__ part,
                                                                     int nparts, size_t pstride,
-                                                                    size_t len, double* __restrict__ out) {
+                                                                    size_t len, int PG,
+                                                                    double* __restrict__ out) {
     __shared__ double acc[kSumWaves][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const size_t e = (size_t)blockIdx.x * 64 + lane;
+    const int p0 = (int)blockIdx.y * PG, p1 = min(nparts, p0 + PG);
     double s = 0.0;
-    if (e < len)
-        for (int p = w; p < nparts; p += kSumWaves) s += part[(size_t)p * pstride + e];
+    if (e < len) {
+#pragma unroll 4
+        for (int p = p0 + w; p < p1; p += kSumWaves) s += part[(size_t)p * pstride + e];
+    }
     acc[w][lane] = s;
     __syncthreads();
     if (w == 0 && e < len) {
         double t = 0.0;
         for (int q = 0; q < kSumWaves; ++q) t += acc[q][lane];
-        out[e] = t;
+        out[(size_t)blockIdx.y * len + e] = t;
     }
 }
 
@@ -753,15 +759,39 @@ static void launch_gj(const MstepArgs& ma, hipStream_t st) {
 
 }  // namespace
 
+/* groups of the first pass: about 4 workgroups per CU in all, at least 32
+ * partials per group, at most kSumGroups */
+static int sum_groups(int nparts, size_t len) {
+    const long xb = (long)((len + 63) / 64);
+    long g = (1024 + xb - 1) / xb;
+    g = std::min<long>(g, std::max(1, nparts / 32));
+    return (int)std::max(1L, std::min<long>(g, kSumGroups));
+}
+
+/* out = sum over nparts partials: one pass, or a pass over groups into tmp
+ * and a second pass over the groups */
+static void sum_parts(const double* part, int nparts, size_t pstride, size_t len, double* tmp,
+                      double* out, hipStream_t st) {
+    const dim3 blk(64 * kSumWaves);
+    const unsigned xb = (unsigned)((len + 63) / 64);
+    const int G = sum_groups(nparts, len);
+    if (G == 1) {
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(xb, 1), blk, 0, st, part, nparts, pstride, len, nparts, out);
+        return;
+    }
+    const int PG = (nparts + G - 1) / G;
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(xb, G), blk, 0, st, part, nparts, pstride, len, PG, tmp);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(xb, 1), blk, 0, st, (const double*)tmp, G, len, len, G, out);
+}
+
 int launch_reduce(const double* part, int nparts, size_t pstride, const double* gxpart,
-                  int nchunks, double* raw, double* gxtot, const MstepArgs& ma, hipStream_t st) {
+                  int nchunks, double* raw, double* gxtot, double* tmp, const MstepArgs& ma,
+                  hipStream_t st) {
     const int NS = kStatsPerJ * ma.m + kStatsPerI;
     const size_t plen = (size_t)ma.M * NS * ma.NP + ma.M + 1;
-    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((plen + 63) / 64)), dim3(64 * kSumWaves), 0,
-                       st, part, nparts, pstride, plen, raw);
+    sum_parts(part, nparts, pstride, plen, tmp, raw, st);
     const size_t glen = (size_t)ma.ldb * ma.NP;
-    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((glen + 63) / 64)), dim3(64 * kSumWaves), 0,
-                       st, gxpart, nchunks, glen, glen, gxtot);
+    sum_parts(gxpart, nchunks, glen, glen, tmp, gxtot, st);
     /* one phi row per workgroup (threads over its columns) */
     hipLaunchKernelGGL(finalize_kernel, dim3(ma.M, ma.n), dim3(kBlk), 0, st, raw, gxtot, ma);
     return hipGetLastError() == hipSuccess ? 0 : -1;
