@@ -199,7 +199,7 @@ struct amica_engine {
             else LCHK(launch_unmix_bf16x3(xw, xbf, Whi, Wlo, wc, b, o.T, NP, ldb, cu_budget, st));
         }
         else
-            LCHK(launch_unmix_f64(xw64, Wg64, wc64, b64, o.T, NP, ldb, st));
+            LCHK(launch_unmix_f64(xw64, Wg64, wc64, b64, o.T, NP, ldb, cu_budget, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
         LCHK(launch_estep(eargs(ltall_on ? ltall : nullptr), o.m, nblocks, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[2], st));

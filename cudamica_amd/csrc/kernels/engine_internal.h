@@ -75,7 +75,7 @@ int estep_blocks_per_cu(int m, bool b64, int threads, bool m1);
 /* GEMMs ------------------------------------------------------------------ */
 /* b[t][c] = sum_k x[t][k] W[c][k] - wc[c]    (c < ldb, k < NP) */
 int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double* b,
-                     long T, int NP, int ldb, hipStream_t st);
+                     long T, int NP, int ldb, int cus, hipStream_t st);
 /* b = x_hi W_hi + x_hi W_lo + x_lo W_hi (three MFMAs, ~2^-16 relative) with
  * W given as bf16 hi and lo, x as the split bf16 copy xbf ([T][NP] hi, then
  * [T][NP] lo) or, for NP > 256, float32 x split on the fly. */

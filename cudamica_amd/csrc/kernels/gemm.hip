@@ -325,6 +325,111 @@ __global__ __launch_bounds__(512) void unmix_lds_kernel(const void* __restrict__
     }
 }
 
+/* FP64 unmixing with register-resident W (the FP32 path's GEMM1): the
+ * layout of unmix_lds_kernel with v_mfma_f64_16x16x4f64.  Wave w of the
+ * workgroup's NT / 64 owns 16 CT columns and keeps W[col][4 ks + l/16] for its
+ * KS = NP / 4 k-steps in registers; 16-sample tiles of x (FP64) come from
+ * HBM kUnmixStages tiles ahead, 16 bytes per lane, through a double-buffered
+ * LDS slot (rows padded by 16 bytes: the 16 lanes reading one A fragment
+ * column hit distinct banks); one barrier per tile.  FP64 MFMA output: register
+ * r of lane l holds sample (l/16) + 4r of column l%16. */
+template <int CT, int KS, int NT>
+__global__ __launch_bounds__(NT) void unmix_f64_lds_kernel(const double* __restrict__ x,
+                                                             const double* __restrict__ Wg,
+                                                             const double* __restrict__ wc,
+                                                             double* __restrict__ b, long T, int ldb,
+                                                             long ntiles) {
+    constexpr int NP = 4 * KS;
+    constexpr int ROWB = 8 * NP;                   /* bytes of one sample row */
+    constexpr int RS = ROWB + 16;
+    constexpr int SLOT = 16 * RS;
+    constexpr int CPR = ROWB / 16;
+    constexpr int CHUNKS = 16 * CPR;
+    constexpr int NCH = (CHUNKS + NT - 1) / NT;
+    constexpr int D = kUnmixStages;
+    __shared__ __align__(16) unsigned char lds[2 * SLOT];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int c0 = (blockIdx.y * (NT / 64) + w) * 16 * CT;
+    const bool computes = c0 < ldb;                /* wave-uniform */
+    double wf[CT][KS];
+    double bias[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const int col = c0 + c * 16 + (lane & 15);
+        bias[c] = computes ? wc[col] : 0.0;
+#pragma unroll
+        for (int k = 0; k < KS; ++k)
+            wf[c][k] = computes ? Wg[(size_t)col * NP + 4 * k + (lane >> 4)] : 0.0;
+    }
+    const long first = blockIdx.x, stride = gridDim.x;
+    const long ntl = first < ntiles ? (ntiles - 1 - first) / stride + 1 : 0;   /* block-uniform */
+
+    auto gload = [&](long j, uint4 (&r)[NCH]) {
+        const long tile = first + j * stride;
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+            const int c = tid + q * NT;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (c < CHUNKS && j < ntl) {
+                const int srow = c / CPR, cc = c - srow * CPR;
+                const long t = tile * 16 + srow;
+                if (t < T) v = *(const uint4*)((const unsigned char*)x + (size_t)t * ROWB + cc * 16);
+            }
+            r[q] = v;
+        }
+    };
+    auto lwrite = [&](int slot, const uint4 (&r)[NCH]) {
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+            const int c = tid + q * NT;
+            if (c < CHUNKS) {
+                const int srow = c / CPR, cc = c - srow * CPR;
+                *(uint4*)(lds + slot * SLOT + srow * RS + cc * 16) = r[q];
+            }
+        }
+    };
+    auto compute = [&](int slot, long j) {
+        const long tile = first + j * stride;
+        const double* base = (const double*)(lds + slot * SLOT + (lane & 15) * RS) + (lane >> 4);
+        f64x4 acc[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[c] = (f64x4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            const double av = base[4 * k];
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+                acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wf[c][k], acc[c], 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const int col = c0 + c * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long t = tile * 16 + (lane >> 4) + 4 * r;
+                if (t < T) b[(size_t)t * ldb + col] = acc[c][r] - bias[c];
+            }
+        }
+    };
+
+    uint4 R[D][NCH];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gload(d, R[d]);
+    for (long j0 = 0; j0 < ntl; j0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            const long j = j0 + u;
+            if (j < ntl) {                         /* block-uniform */
+                lwrite((int)(j & 1), R[u]);
+                gload(j + D, R[u]);
+                __syncthreads();
+                if (computes) compute((int)(j & 1), j);
+            }
+        }
+    }
+}
+
 /* ---------------- statistic GEMM G = g^T x, FP32 --------------------- */
 /* FP32 statistic GEMM in slabs: workgroup = ldb/64 waves over one chunk of
  * samples (at most a few thousand, so the FP32 accumulation inside the MFMA
@@ -470,8 +575,45 @@ __global__ void pack8_kernel(const uint16_t* __restrict__ xhi, long T, int NP,
 }  // namespace
 
 int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double* b, long T,
-                     int NP, int ldb, hipStream_t st) {
+                     int NP, int ldb, int cus, hipStream_t st) {
     if (NP % 32 || ldb % 64) return -1;
+    /* register-resident W: 16 CT columns x NP inputs per wave, 2 CT NP / 4 VGPRs
+     * (at most 128), workgroups of NT threads whose waves all compute
+     * (ldb >= NT / 4 CT); other shapes use the LDS-tiled kernel below */
+    const int ctmax = NP == 64 ? 4 : NP == 128 ? 2 : NP == 192 ? 1 : 0;
+    int CT = 0, NT = 0;
+    for (int nt = 512; nt >= 256 && !CT; nt >>= 1)
+        for (int ct = ctmax; ct >= 1; ct >>= 1)
+            if (ldb >= nt / 4 * ct && ldb % (16 * ct) == 0) { CT = ct; NT = nt; break; }
+    if (CT) {
+        const int cols = NT / 4 * CT;                  /* columns per workgroup */
+        const int groups = (ldb + cols - 1) / cols;
+        const long ntiles = (T + 15) / 16;
+#define AMICA_UNMIX_F64(C, K, N)                                                                \
+        do {                                                                                    \
+            static int bpc = 0;                                                                 \
+            if (bpc == 0) {                                                                     \
+                int nb = 0;                                                                     \
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                              \
+                        &nb, unmix_f64_lds_kernel<C, K, N>, N, 0) != hipSuccess || nb < 1)     \
+                    nb = 1;                                                                     \
+                bpc = nb;                                                                       \
+            }                                                                                   \
+            const long gx = std::max(1L, std::min<long>(ntiles, (long)cus * bpc / groups));     \
+            hipLaunchKernelGGL((unmix_f64_lds_kernel<C, K, N>), dim3((unsigned)gx, groups),     \
+                               dim3(N), 0, st, x, Wg, wc, b, T, ldb, ntiles);                  \
+        } while (0)
+        if (NP == 64 && CT == 4) AMICA_UNMIX_F64(4, 16, 512);
+        else if (NP == 64 && CT == 2) AMICA_UNMIX_F64(2, 16, 512);
+        else if (NP == 64 && NT == 512) AMICA_UNMIX_F64(1, 16, 512);
+        else if (NP == 64) AMICA_UNMIX_F64(1, 16, 256);
+        else if (NP == 128 && CT == 2) AMICA_UNMIX_F64(2, 32, 512);
+        else if (NP == 128 && NT == 512) AMICA_UNMIX_F64(1, 32, 512);
+        else if (NP == 192 && NT == 512) AMICA_UNMIX_F64(1, 48, 512);
+        else CT = 0;
+#undef AMICA_UNMIX_F64
+        if (CT) return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
     dim3 grid((unsigned)((T + 63) / 64), ldb / 64);
     hipLaunchKernelGGL(unmix_f64_kernel, grid, dim3(256), 0, st, x, Wg, wc, b, T, NP, ldb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
