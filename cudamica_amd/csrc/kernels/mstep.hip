@@ -456,9 +456,8 @@ __device__ __forceinline__ void set_ldet(const MstepArgs& a, int h, double lad) 
  *   a'[i][j] = (i in P ? 0 : a[i][j]) + sum_q a'[i][k0 + q] a[p_q][j],
  * which is the Schur complement form of the same elimination.  A panel is
  * factored by one wave with its n x NB values in registers (lane l holds
- * rows l + 64 r), so a step costs no barrier beyond the pivot row's trip
- * through LDS; the trailing update runs one wave per column over the whole
- * GPU.  Every panel after the first is factored by block 0 of the launch
+ * rows l + 64 r), so a step needs no barrier and no LDS; the trailing
+ * update runs one wave per column over the whole GPU.  Every panel after the first is factored by block 0 of the launch
  * that applies the previous panel, right after it has updated the new
  * panel's columns.
  *
@@ -526,11 +525,26 @@ __device__ __forceinline__ int wave_argmax(double bv, int bi) {
     return besti;
 }
 
-/* NB Gauss-Jordan steps on the panel v (columns k0 .., rows lane + 64 r),
- * one wave.  rowbuf: NB doubles of LDS. */
+/* argmax of |v[.][c]| over this wave's unused rows (the pivot of column c) */
 template <int R, int NB>
-__device__ __forceinline__ void gj_factor(const GjView& g, int n, int k0, double (&v)[R][NB],
-                                          double* rowbuf) {
+__device__ __forceinline__ int gj_col_argmax(const double (&v)[R][NB], const int (&st)[R], int c) {
+    double bv = -1.0;
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double av = fabs(v[r][c]);
+        if (av != av) av = __builtin_inf();         /* a NaN column still names an unused row */
+        if (st[r] < 0 && av > bv) { bv = av; bi = (int)threadIdx.x + 64 * r; }
+    }
+    return wave_argmax(bv, bi);
+}
+
+/* NB Gauss-Jordan steps on the panel v (columns k0 .., rows lane + 64 r),
+ * one wave, no LDS: the pivot row reaches every lane by readlane, and each
+ * step updates the next pivot column first and takes its argmax before the
+ * other columns, so that the reduction overlaps their FMAs. */
+template <int R, int NB>
+__device__ __forceinline__ void gj_factor(const GjView& g, int n, int k0, double (&v)[R][NB]) {
     const int lane = threadIdx.x;
     const int nc = min(NB, n - k0);
     int st[R];
@@ -539,51 +553,60 @@ __device__ __forceinline__ void gj_factor(const GjView& g, int n, int k0, double
         const int i = lane + 64 * r;
         st[r] = i < n ? g.step[i] : n;          /* padding rows count as used */
     }
+    int p = gj_col_argmax<R, NB>(v, st, 0);
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
         if (c >= nc) continue;                      /* wave-uniform; keeps the loop unrollable */
         const int k = k0 + c;
-        double bv = -1.0;
-        int bi = 0x7fffffff;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            double av = fabs(v[r][c]);
-            if (av != av) av = __builtin_inf();     /* a NaN column still names an unused row */
-            if (st[r] < 0 && av > bv) { bv = av; bi = lane + 64 * r; }
-        }
-        const int p = wave_argmax(bv, bi);
         const int pl = p & 63, pr = p >> 6;
+        double prow[NB];
+#pragma unroll
+        for (int c2 = 0; c2 < NB; ++c2) prow[c2] = 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (r == pr && lane == pl) {
+            if (r == pr) {                          /* wave-uniform */
 #pragma unroll
-                for (int c2 = 0; c2 < NB; ++c2) rowbuf[c2] = v[r][c2];
+                for (int c2 = 0; c2 < NB; ++c2) prow[c2] = readlane_f64(v[r][c2], pl);
             }
-        __syncthreads();
-        const double d = rowbuf[c];
+        const double d = prow[c];
         /* 1/d: hardware estimate and two Newton steps (within an ulp) */
         double inv = __builtin_amdgcn_rcp(d);
         inv = fma(fma(-d, inv, 1.0), inv, inv);
         inv = fma(fma(-d, inv, 1.0), inv, inv);
         double prw[NB];
 #pragma unroll
-        for (int c2 = 0; c2 < NB; ++c2) prw[c2] = (c2 == c) ? inv : rowbuf[c2] * inv;
+        for (int c2 = 0; c2 < NB; ++c2) prw[c2] = (c2 == c) ? inv : prow[c2] * inv;
+        double pc[R];
+        bool isp[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const double pc = v[r][c];
-            const bool isp = lane + 64 * r == p;
+            pc[r] = v[r][c];
+            isp[r] = lane + 64 * r == p;
+            if (isp[r]) st[r] = k;
+        }
+        /* the next pivot column first, then its argmax */
+        int pn = 0;
+        if (c + 1 < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double nv = v[r][c + 1] - pc[r] * prw[c + 1];
+                v[r][c + 1] = isp[r] ? prw[c + 1] : nv;
+            }
+            if (c + 1 < nc) pn = gj_col_argmax<R, NB>(v, st, c + 1);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
 #pragma unroll
             for (int c2 = 0; c2 < NB; ++c2) {
-                const double nv = ((c2 == c) ? 0.0 : v[r][c2]) - pc * prw[c2];
-                v[r][c2] = isp ? prw[c2] : nv;
+                if (c2 == c + 1) continue;
+                const double nv = ((c2 == c) ? 0.0 : v[r][c2]) - pc[r] * prw[c2];
+                v[r][c2] = isp[r] ? prw[c2] : nv;
             }
-            if (isp) st[r] = k;
-        }
         if (lane == 0) {
             g.piv[k] = p;
             g.dk[k] = fabs(d);
         }
-        __syncthreads();                            /* rowbuf is rewritten next step */
+        p = pn;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -623,7 +646,6 @@ template <int R, int NB>
 __global__ __launch_bounds__(64) void gj_panel_kernel(MstepArgs a) {
     const GjView g = gj_view(a, blockIdx.x);
     const int n = a.n;
-    __shared__ double rowbuf[NB];
     double v[R][NB];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -632,7 +654,7 @@ __global__ __launch_bounds__(64) void gj_panel_kernel(MstepArgs a) {
         for (int c2 = 0; c2 < NB; ++c2)
             v[r][c2] = (i < n && c2 < n) ? g.aw[i + (size_t)c2 * n] : 0.0;
     }
-    gj_factor<R, NB>(g, n, 0, v, rowbuf);
+    gj_factor<R, NB>(g, n, 0, v);
     gj_store<R, NB>(g, n, 0, v);
 }
 
@@ -645,7 +667,6 @@ __global__ __launch_bounds__(64) void gj_update_kernel(MstepArgs a, int kp, int 
     const GjView g = gj_view(a, h);
     const int ncp = min(NB, n - kp);
     __shared__ int pv[NB];
-    __shared__ double rowbuf[NB];
     if (lane < ncp) pv[lane] = g.piv[kp + lane];
     __syncthreads();
     bool inP[R];
@@ -682,7 +703,7 @@ __global__ __launch_bounds__(64) void gj_update_kernel(MstepArgs a, int kp, int 
 #pragma unroll
                 for (int c2 = 0; c2 < NB; ++c2) v[r][c2] += pq[r] * o[c2];
         }
-        gj_factor<R, NB>(g, n, kn, v, rowbuf);
+        gj_factor<R, NB>(g, n, kn, v);
         gj_store<R, NB>(g, n, kn, v);
         return;
     }
