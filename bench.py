@@ -10,7 +10,10 @@ reparameterisation) over all samples, which stay resident in HBM.
 
 Prints one JSON line.  Fields beyond the driver's contract:
   roofline      HBM roofline of the fused E-step, from HIP events recorded
-                around it on the engine stream during the timed steps
+                around it on the engine stream during the timed steps; also
+                against the bandwidth of a measured device-to-device copy
+  gemm          algorithmic GEMM FLOPs (4 M n^2 T) over the two GEMMs' time,
+                against the dense MFMA peak of their operand type
   cpu_baseline  the FP64 CPU oracle on a bounded sample of the workload
   ll_rel_err_vs_fp64  max over the first iterations on that sample (same
                 data, same initial state) of |LL_gpu - LL_fp64| / |LL_fp64|;
@@ -87,6 +90,41 @@ def cpu_baseline(w, x, st0, precision, iters=2, target_s=15.0):
     return {"value": its_full, "unit": "EM iters/s", "cores": cores, "kind": "port",
             "sample": f"FP64 oracle, {iters} EM iterations on the first {Ts} of {w.T} samples, "
                       f"{dt:.1f} s, scaled by T/{Ts}"}, rel, Ts
+
+
+# dense MFMA peaks of the GEMMs' operand type (MI355X, TFLOP/s): split-BF16
+# and FP8 paths run bf16 MFMAs, the FP32 path FP64 (unmixing) and FP32
+GEMM_PEAK_TFLOPS = {16: 2500.0, 8: 2500.0, 32: 78.6}
+
+
+def measured_hbm_gbs(mib=2048, reps=5):
+    """Achievable HBM bandwidth: the best of `reps` device-to-device copies
+    of `mib` MiB (bytes read + bytes written per second), timed with HIP
+    events through the HIP runtime the engine uses."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    ok = (hip.hipMalloc(ctypes.byref(a), nbytes) == 0 and hip.hipMalloc(ctypes.byref(b), nbytes) == 0
+          and hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+          and hip.hipMemset(a, 1, nbytes) == 0)
+    best = 0.0
+    for _ in range(reps if ok else 0):
+        ms = ctypes.c_float(0.0)
+        if (hip.hipEventRecord(e0, None) != 0
+                or hip.hipMemcpyAsync(b, a, nbytes, 3, None) != 0     # device to device
+                or hip.hipEventRecord(e1, None) != 0 or hip.hipEventSynchronize(e1) != 0
+                or hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) != 0 or ms.value <= 0):
+            break
+        best = max(best, 2.0 * nbytes.value / (ms.value * 1e-3) / 1e9)
+    for p in (a, b):
+        if p.value:
+            hip.hipFree(p)
+    for e in (e0, e1):
+        if e.value:
+            hip.hipEventDestroy(e)
+    return best or None
 
 
 def copy_probe(mib):
@@ -220,11 +258,15 @@ def main():
     eng.close()
     its = args.steps / dt
     log(f"timed {args.steps} iterations: {dt * 1e3 / args.steps:.3f} ms/iter; kernels (ms) "
-        f"prepare+unmix {kms[0]:.3f} estep {kms[1]:.3f} gx {kms[2]:.3f} reduce+mstep {kms[3]:.3f}; "
-        f"LL {lg['ll'][-1]:.10f}")
+        f"inverse {kms[0]:.3f} unmix {kms[1]:.3f} estep {kms[2]:.3f} gx {kms[3]:.3f} "
+        f"reduce+mstep {kms[4]:.3f}; LL {lg['ll'][-1]:.10f}")
 
     nbytes = estep_bytes(n, T, M, prec)
-    achieved = nbytes / (kms[1] * 1e-3) / 1e9
+    achieved = nbytes / (kms[2] * 1e-3) / 1e9
+    hbm = measured_hbm_gbs()
+    gemm_flop = 4.0 * M * n * n * T          # GEMM1 + GEMM2, algorithmic
+    gemm_s = (kms[1] + kms[3]) * 1e-3
+    gemm_peak = GEMM_PEAK_TFLOPS[prec]
     out = {
         "metric": METRIC,
         "value": its,
@@ -243,9 +285,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(w.name, prec),
                      "kernel": "estep_kernel", "bytes_per_launch": nbytes,
-                     "avg_ms": kms[1], "launches": cnt},
-        "kernel_ms": {"prepare_unmix": kms[0], "estep": kms[1], "stat_gemm": kms[2],
-                      "reduce_mstep": kms[3]},
+                     "avg_ms": kms[2], "launches": cnt,
+                     "measured_copy_gbs": hbm,
+                     "frac_of_measured": achieved / hbm if hbm else None},
+        "gemm": {"flop_per_iter": gemm_flop, "ms": gemm_s * 1e3,
+                 "tflops": gemm_flop / gemm_s / 1e12,
+                 "peak_tflops": gemm_peak, "mfma_frac": gemm_flop / gemm_s / 1e12 / gemm_peak},
+        "kernel_ms": {"inverse": kms[0], "unmix": kms[1], "estep": kms[2], "stat_gemm": kms[3],
+                      "reduce_mstep": kms[4]},
         "ll_last": float(lg["ll"][-1]),
     }
     if not args.no_cpu_baseline:

@@ -128,7 +128,7 @@ struct amica_engine {
     hipGraph_t graph = nullptr;
     /* timing: one set of 5 events per timed iteration, read back lazily */
     int timing = 0;
-    std::vector<std::array<hipEvent_t, 5>> evs;
+    std::vector<std::array<hipEvent_t, 6>> evs;
     size_t ev_used = 0;
     hipEvent_t* ev = nullptr;
 
@@ -193,6 +193,7 @@ struct amica_engine {
         const MstepArgs ma = margs();
         if (with_timing) HIPCHK(hipEventRecord(ev[0], st));
         LCHK(launch_prepare(ma, st));
+        if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
         if (bf16)
         {
             if (fp8) LCHK(launch_unmix_fp8(xq, Whi, Wlo, wc, b, o.T, NP, ldb, cu_budget, st));
@@ -200,14 +201,14 @@ struct amica_engine {
         }
         else
             LCHK(launch_unmix_f64(xw64, Wg64, wc64, b64, o.T, NP, ldb, cu_budget, st));
-        if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
-        LCHK(launch_estep(eargs(ltall_on ? ltall : nullptr), o.m, nblocks, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[2], st));
+        LCHK(launch_estep(eargs(ltall_on ? ltall : nullptr), o.m, nblocks, st));
+        if (with_timing) HIPCHK(hipEventRecord(ev[3], st));
         if (bf16)
             LCHK(launch_gx_bf16((const uint16_t*)g, x8, gxpart, o.T, NP, ldb, nchunks, st));
         else
             LCHK(launch_gx_f32((const float*)g, xw, gxpart, o.T, NP, ldb, nchunks, st));
-        if (with_timing) HIPCHK(hipEventRecord(ev[3], st));
+        if (with_timing) HIPCHK(hipEventRecord(ev[4], st));
         LCHK(launch_reduce(part, nblocks, pstride, gxpart, nchunks, raw, gxtot, sumtmp, ma, st));
         return 0;
     }
@@ -216,13 +217,13 @@ struct amica_engine {
         int rc = enqueue_estep(with_timing);
         if (rc) return rc;
         LCHK(launch_mstep(margs(), st));
-        if (with_timing) HIPCHK(hipEventRecord(ev[4], st));
+        if (with_timing) HIPCHK(hipEventRecord(ev[5], st));
         return 0;
     }
 
     int next_event_set() {
         if (ev_used == evs.size()) {
-            std::array<hipEvent_t, 5> s;
+            std::array<hipEvent_t, 6> s;
             for (auto& v : s) HIPCHK(hipEventCreate(&v));
             evs.push_back(s);
         }
@@ -772,16 +773,16 @@ int amica_engine_get_timing(amica_engine* e, double* ms, int* count) {
     if (!e || !ms) return fail(AMICA_E_INVALID, "null argument");
     HIPCHK(hipSetDevice(e->o.device));
     HIPCHK(hipStreamSynchronize(e->st));
-    double sum[4] = {0, 0, 0, 0};
+    double sum[5] = {0, 0, 0, 0, 0};
     for (size_t s = 0; s < e->ev_used; ++s) {
         const auto& v = e->evs[s];
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 5; ++k) {
             float t = 0.f;
             HIPCHK(hipEventElapsedTime(&t, v[k], v[k + 1]));
             sum[k] += t;
         }
     }
-    for (int k = 0; k < 4; ++k) ms[k] = e->ev_used ? sum[k] / (double)e->ev_used : 0.0;
+    for (int k = 0; k < 5; ++k) ms[k] = e->ev_used ? sum[k] / (double)e->ev_used : 0.0;
     if (count) *count = (int)e->ev_used;
     return 0;
 }

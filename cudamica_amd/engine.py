@@ -279,7 +279,9 @@ class Engine:
         _check(self.L.amica_engine_set_timing(self.h, 1 if on else 0), "set_timing")
 
     def timing(self):
-        ms = np.zeros(4)
+        """Average ms per timed iteration: inverse + constants, unmixing GEMM,
+        E-step, statistic GEMM, reductions + M-step."""
+        ms = np.zeros(5)
         cnt = np.zeros(1, np.int32)
         _check(self.L.amica_engine_get_timing(self.h, _d(ms), _i(cnt)), "get_timing")
         return ms, int(cnt[0])

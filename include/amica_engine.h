@@ -121,9 +121,10 @@ int  amica_engine_get_ltall(amica_engine* eng, double* ltall);
 int  amica_engine_output(amica_engine* eng, double* A_out, double* c_out);
 
 /* Average duration (ms) of each kernel class over the last timed
- * iterations, measured with HIP events on the engine stream:
- * [0] unmixing GEMM, [1] fused E-step, [2] statistic GEMM, [3] reductions +
- * M-step.  Enabled by amica_engine_set_timing(eng, 1). */
+ * iterations, measured with HIP events on the engine stream (ms[5]):
+ * [0] W = A^-1 and the E-step constants, [1] unmixing GEMM, [2] fused
+ * E-step, [3] statistic GEMM, [4] reductions + M-step.  Enabled by
+ * amica_engine_set_timing(eng, 1). */
 int  amica_engine_set_timing(amica_engine* eng, int on);
 int  amica_engine_get_timing(amica_engine* eng, double* ms, int* count);
 
