@@ -15,6 +15,9 @@ Prints one JSON line.  Fields beyond the driver's contract:
   gemm          algorithmic GEMM FLOPs (4 M n^2 T) over the two GEMMs' time,
                 against the dense MFMA peak of their operand type
   cpu_baseline  the FP64 CPU oracle on a bounded sample of the workload
+  ll_vs_fp32_path     BF16 / FP8 runs: the relative LL difference to the FP32
+                path (FP64 unmixing, held to the oracle by the parity tests)
+                after 50 iterations on the whole recording
   ll_rel_err_vs_fp64  max over the first iterations on that sample (same
                 data, same initial state) of |LL_gpu - LL_fp64| / |LL_fp64|;
                 ll_rel_err_per_iteration lists them (the first is the
@@ -127,6 +130,47 @@ def measured_hbm_gbs(mib=2048, reps=5):
     return best or None
 
 
+def ll_vs_fp32_path(w, x, st0, precision, iters=50):
+    """The reduced-precision path against the FP32 path (FP64 unmixing, held
+    to the FP64 oracle by the parity tests) on the whole recording, both from
+    the same state for `iters` iterations.  The FP8 path is compared with the
+    FP32 path on its own dequantised data (the data it runs on).  Returns the
+    relative LL difference at the last iteration, its maximum over the run
+    and where that was, and both runs' counts of LL-decrease iterations."""
+    import cudamica_amd as ca
+    logs = []
+    with ca.Engine(w.options(precision=precision, max_iter=iters)) as eng:
+        eng.load_raw(x)
+        eng.preprocess(w.do_sphere)
+        if precision == 8:
+            xh = eng.white().astype(np.float64)
+            ldetS = eng.sphere()[2]
+        eng.set_state(st0)
+        eng.iterate(iters, graph=True)
+        eng.sync()
+        logs.append(eng.log(iters))
+    with ca.Engine(w.options(precision=32, max_iter=iters)) as eng:
+        if precision == 8:
+            eng.load_white(xh, ldetS)
+            del xh
+        else:
+            eng.load_raw(x)
+            eng.preprocess(w.do_sphere)
+        eng.set_state(st0)
+        eng.iterate(iters, graph=True)
+        eng.sync()
+        logs.append(eng.log(iters))
+    rel = np.abs(logs[0]["ll"] - logs[1]["ll"]) / np.abs(logs[1]["ll"])
+    imax = int(np.argmax(rel))
+    log(f"LL vs the FP32 path over {iters} iterations: last {rel[-1]:.2e}, max {rel[imax]:.2e} "
+        f"(iteration {imax + 1})")
+    return {"iterations": iters, "rel_last": float(rel[-1]), "rel_max": float(rel[imax]),
+            "rel_max_iteration": imax + 1,
+            "reference": "fp32 path on the dequantised data" if precision == 8 else "fp32 path",
+            "lldec_iterations": [int(np.sum(l["lldec"])) for l in logs],
+            "ll_last": float(logs[0]["ll"][-1]), "ll_last_fp32": float(logs[1]["ll"][-1])}
+
+
 def copy_probe(mib):
     """One hipMemset and one device-to-device hipMemcpy of `mib` MiB (the HIP
     runtime's fill and copy kernels), so that a rocprofv3 --pmc pass over this
@@ -212,6 +256,8 @@ def main():
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default=None)
     ap.add_argument("--T", type=int, default=None, help="override the sample count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ll-check", action="store_true",
+                    help="skip the 50-iteration LL comparison with the FP32 path")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a memset + copy of MIB MiB first (counter calibration)")
     args = ap.parse_args()
@@ -295,6 +341,8 @@ def main():
                       "reduce_mstep": kms[4]},
         "ll_last": float(lg["ll"][-1]),
     }
+    if prec != 32 and not args.no_ll_check:
+        out["ll_vs_fp32_path"] = ll_vs_fp32_path(w, x, st0, prec)
     if not args.no_cpu_baseline:
         cb, rel, Ts = cpu_baseline(w, x, st0, prec)
         out["cpu_baseline"] = cb

@@ -81,6 +81,18 @@ __device__ __forceinline__ float row_sum_lane15(float v) {
     return v;
 }
 
+/* Floor of |y| before log2|y|.  FP32 path (y formed in FP64 from FP64 b):
+ * the smallest normal float.  Split-BF16 and FP8 paths (y = sb b - sb mu in
+ * FP32 from FP32 b): 2^-24, the FP32 spacing at |sb mu| ~ 1.  There b can
+ * round exactly onto mu, and the y that cancels to 0 would give
+ * |y|^(rho-2) = 2^63 for rho = 1.5 and swamp the mu denominator
+ * sum(u |y|^(rho-2)), which is singular at y = 0 for rho < 2; at the FP32
+ * spacing the weight stays of the size FP64 gives its smallest |y|.  (A
+ * floor at the ~2^-17 error of the BF16 GEMM would cap the weights of all
+ * |y| below it and bias mu_denom low.) */
+template <bool B64>
+__device__ __forceinline__ constexpr float y_floor() { return B64 ? 1.17549435e-38f : 0x1p-24f; }
+
 /* two floats -> two round-to-nearest-even bf16 in one dword (a low) */
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)a) |
@@ -207,7 +219,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 float y;
                 if constexpr (B64) y = (float)fma(sb64[j], BV64[s], nsbmu64[j]);
                 else y = fmaf(sb[j], BV[s], nsbmu[j]);
-                const float la = fast_log2(fmaxf(fabsf(y), 1.17549435e-38f));
+                const float la = fast_log2(fmaxf(fabsf(y), y_floor<B64>()));
                 const float ym2 = fast_exp2(rm2[j] * la);
                 const float sy = ym2 * y;
                 const float yr = sy * y;
@@ -458,7 +470,7 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
             float y;
             if constexpr (B64) y = (float)fma(sb64[j], dcur, nsbmu64[j]);
             else y = fmaf(sb[j], bv, nsbmu[j]);
-            LA[j] = fast_log2(fmaxf(fabsf(y), 1.17549435e-38f));
+            LA[j] = fast_log2(fmaxf(fabsf(y), y_floor<B64>()));
             YM2[j] = fast_exp2(rm2[j] * LA[j]);
             SY[j] = YM2[j] * y;
             YR[j] = SY[j] * y;

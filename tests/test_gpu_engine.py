@@ -145,6 +145,40 @@ def _run_both(w, T, iters, precision, newt=None, graph=False):
     return fin, logs, gst, glog
 
 
+def test_bf16_mu_denom_at_exact_cancellation(built):
+    """At the workload's initial state mu is exactly -1, 0 or 1, so over
+    ~10^9 elements some b round onto mu in FP32 and y cancels to 0; with
+    rho = 1.5 the weight |y|^(rho-2) of such an element must stay at the
+    BF16 path's resolution (2^-17) instead of 2^63.  The split-BF16
+    statistics are compared with the FP32 path (FP64 unmixing, held to the
+    oracle by the tests above): mu_denom within 5e-2 (its error scales like
+    sqrt of b's rounding there), mu after one iteration within 1e-2."""
+    w = ca.WORKLOADS["n128_T2M_M4"]
+    n, M, m, T = w.n, w.M, w.m, 400000
+    _say(f"[bf16 mu_denom] n={n} T={T} M={M} m={m}, initial state of the workload")
+    x = ca.synth_data(w.spec(T=T))
+    st0 = w.init_state()
+    res = {}
+    for p in (ca.PREC_BF16, ca.PREC_FP32):
+        with ca.Engine(w.options(T=T, precision=p, max_iter=2)) as eng:
+            eng.load_raw(x)
+            eng.preprocess(w.do_sphere)
+            eng.set_state(st0)
+            stats = eng.estep_stats()
+            eng.iterate(1)
+            eng.sync()
+            res[p] = (stats, eng.get_state())
+    a, _ = po.stats_views(res[ca.PREC_BF16][0], n, m, M)
+    b, _ = po.stats_views(res[ca.PREC_FP32][0], n, m, M)
+    md = max(np.linalg.norm(a[h]["mu_denom"] - b[h]["mu_denom"]) / np.linalg.norm(b[h]["mu_denom"])
+             for h in range(M))
+    mu = (np.linalg.norm(res[ca.PREC_BF16][1]["mu"] - res[ca.PREC_FP32][1]["mu"])
+          / np.linalg.norm(res[ca.PREC_FP32][1]["mu"]))
+    _say(f"  mu_denom relative difference {md:.2e}; mu after one iteration {mu:.2e}")
+    assert md <= 5e-2
+    assert mu <= 1e-2
+
+
 def _oracle_envelope(w, T, iters, newt, fin, logs, seeds=(1, 2), eps=6e-8):
     """Conditioning of the EM trajectory itself: the FP64 oracle rerun on its
     whitened input perturbed by float32-rounding-sized noise (once rounded to
