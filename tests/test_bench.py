@@ -1,0 +1,27 @@
+"""bench.py's contract pieces that need no GPU: the algorithmic E-step bytes
+per launch (M n T (b read + g written)) against the worked values of the
+measurement plan (64 ch x 1M: 0.51 GB FP32; 128 ch x 2M x 4 models: 8.19 GB
+FP32, 6.14 GB with bf16 g; 256 ch x 4M x 2 models: 16.4 / 12.3 GB), and the
+single-GPU refusal of --gpus N > 1."""
+import os
+import subprocess
+import sys
+
+import bench
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_estep_bytes_match_the_worked_values():
+    assert bench.estep_bytes(64, 1_000_000, 1, 32) == 512_000_000
+    assert bench.estep_bytes(128, 2_000_000, 4, 32) == 8_192_000_000
+    assert bench.estep_bytes(128, 2_000_000, 4, 16) == 6_144_000_000
+    assert bench.estep_bytes(256, 4_000_000, 2, 32) == 16_384_000_000
+    assert bench.estep_bytes(256, 4_000_000, 2, 8) == 12_288_000_000
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
