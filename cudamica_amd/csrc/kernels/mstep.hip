@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
     const long nm = (long)n * m;
     if (blockIdx.y == 0) {
         for (long q = threadIdx.x; q < nm; q += blockDim.x) {
-            const int j = (int)(q / n), i = (int)(q % n);
+            const int j = (int)((unsigned)q / (unsigned)n), i = (int)((unsigned)q % (unsigned)n);
             const double be = a.beta[(size_t)h * nm + q], r = a.rho[(size_t)h * nm + q];
             const double sb = sqrt(be);
             const double* rj = rh + (size_t)i * NS + j * kStatsPerJ;
@@ -136,28 +136,41 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
             }
         }
     }
-    /* phi_h[i][k] = sum_l G[h*NP+i][l] W~_h[k][l] - gsum_h[i] wc_h[k] */
+    /* phi_h[i][k] = sum_l G[h*NP+i][l] W~_h[k][l] - gsum_h[i] wc_h[k]: one
+     * wave per (i, k), lanes over l (coalesced rows of W~), two k at a time */
     const double* gsum_base = rh + kStatsPerJ * m + kRawGsum;
     const int rows_per = (n + gridDim.y - 1) / gridDim.y;
     const int ib = blockIdx.y * rows_per, ie = min(n, ib + rows_per);
-    const long cnt = (long)(ie - ib) * n;
-    for (long q = threadIdx.x; q < cnt; q += blockDim.x) {
-        const int i = ib + (int)(q % (ie - ib)), k = (int)(q / (ie - ib));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int i = ib; i < ie; ++i) {
         const double* grow = gx + ((size_t)h * NP + i) * NP;
-        const size_t wrow = ((size_t)h * NP + k) * NP;
-        double s = 0.0, wck;
-        if (a.prec_bf16) {
-            /* the unmixing GEMM used W~ = W_hi + W_lo */
-            for (int l = 0; l < n; ++l)
-                s += grow[l] * ((double)bf16_bits_to_f32(a.Whi[wrow + l]) +
-                                (double)bf16_bits_to_f32(a.Wlo[wrow + l]));
-            wck = (double)a.wc[(size_t)h * NP + k];
-        } else {
-            for (int l = 0; l < n; ++l) s += grow[l] * a.Wg64[wrow + l];
-            wck = a.wc64[(size_t)h * NP + k];
+        const double gs = gsum_base[(size_t)i * NS];
+        for (int k0 = 2 * wv; k0 < n; k0 += 2 * nw) {
+            const int k1 = min(k0 + 1, n - 1);
+            const size_t w0 = ((size_t)h * NP + k0) * NP, w1 = ((size_t)h * NP + k1) * NP;
+            double s0 = 0.0, s1 = 0.0;
+            for (int l = lane; l < n; l += 64) {
+                const double g = grow[l];
+                if (a.prec_bf16) {
+                    /* the unmixing GEMM used W~ = W_hi + W_lo */
+                    s0 += g * ((double)bf16_bits_to_f32(a.Whi[w0 + l]) + (double)bf16_bits_to_f32(a.Wlo[w0 + l]));
+                    s1 += g * ((double)bf16_bits_to_f32(a.Whi[w1 + l]) + (double)bf16_bits_to_f32(a.Wlo[w1 + l]));
+                } else {
+                    s0 += g * a.Wg64[w0 + l];
+                    s1 += g * a.Wg64[w1 + l];
+                }
+            }
+            s0 = wave_sum_f64(s0);
+            s1 = wave_sum_f64(s1);
+            if (lane == 0) {
+                const double wc0 = a.prec_bf16 ? (double)a.wc[(size_t)h * NP + k0] : a.wc64[(size_t)h * NP + k0];
+                st[o.phi + i + (long)k0 * n] = s0 - gs * wc0;
+                if (k1 != k0) {
+                    const double wc1 = a.prec_bf16 ? (double)a.wc[(size_t)h * NP + k1] : a.wc64[(size_t)h * NP + k1];
+                    st[o.phi + i + (long)k1 * n] = s1 - gs * wc1;
+                }
+            }
         }
-        s -= gsum_base[(size_t)i * NS] * wck;
-        st[o.phi + i + (long)k * n] = s;
     }
 }
 
@@ -166,7 +179,8 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
 /* ------------------------------------------------------------------------ */
 
 /* Per model: gm, alpha, sigma2, cnew, phi/vsum, kappa, lambda, mu, beta,
- * rho, the Newton test and the matrix B. */
+ * rho, the Newton test and the matrix B.  (Flat indices below n n <= 2^20
+ * are split with 32-bit division, here and in the kernels below.) */
 __global__ __launch_bounds__(kBlk) void mstep_a_kernel(MstepArgs a) {
     __shared__ double sh[16];
     const int h = blockIdx.x, n = a.n, m = a.m;
@@ -226,7 +240,7 @@ __global__ __launch_bounds__(kBlk) void mstep_a_kernel(MstepArgs a) {
     /* getbflag: D_ik = kappa_i sigma2_i kappa_k sigma2_k <= 1, diagonal included */
     double cnt = 0.0;
     for (long q = threadIdx.x; q < (long)n * n; q += blockDim.x) {
-        const int i = (int)(q % n), k = (int)(q / n);
+        const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
         const double D = (kap[i] * sig[i]) * (kap[k] * sig[k]);
         if (D <= 1.0) cnt += 1.0;
     }
@@ -234,7 +248,7 @@ __global__ __launch_bounds__(kBlk) void mstep_a_kernel(MstepArgs a) {
     const int bflag = (int)bsum;
     const int newt = (bflag == 0 && a.do_newton == 1 && k_it >= a.newt_start_iter) ? 1 : 0;
     for (long q = threadIdx.x; q < (long)n * n; q += blockDim.x) {
-        const int i = (int)(q % n), k = (int)(q / n);
+        const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
         double bv;
         if (newt) {
             if (i == k) {
@@ -303,7 +317,7 @@ __global__ __launch_bounds__(256) void ab_kernel(MstepArgs a) {
     const long nn = (long)n * n;
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (!a.update_A || q >= nn) return;
-    const int i = (int)(q % n), k = (int)(q / n);
+    const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
     const double* A = a.A + (size_t)h * nn;
     const double* B = a.B + (size_t)h * nn + (long)k * n;
     double s = 0.0;
@@ -381,7 +395,7 @@ __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
     const long total = nn + n + nm;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
         if (q < nn) {
-            const int i = (int)(q % n), k = (int)(q / n);
+            const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
             const double wv2 = W[q];
             const size_t dst = ((size_t)h * NP + i) * NP + k;
             if (a.prec_bf16) {
@@ -412,7 +426,7 @@ __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
             else a.wc64[(size_t)h * NP + i] = s;
         } else {
             const long e = q - nn - n;
-            const int j = (int)(e / n), i = (int)(e % n);
+            const int j = (int)((unsigned)e / (unsigned)n), i = (int)((unsigned)e % (unsigned)n);
             const double al = a.alpha[(size_t)h * nm + e], mv = a.mu[(size_t)h * nm + e];
             const double be = a.beta[(size_t)h * nm + e], r = a.rho[(size_t)h * nm + e];
             const double sb = sqrt(be);
