@@ -591,21 +591,18 @@ __device__ __forceinline__ void gj_factor(const GjView& g, int n, int k0, double
 #pragma unroll
         for (int c2 = 0; c2 < NB; ++c2) prw[c2] = (c2 == c) ? inv : prow[c2] * inv;
         double pc[R];
-        bool isp[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             pc[r] = v[r][c];
-            isp[r] = lane + 64 * r == p;
-            if (isp[r]) st[r] = k;
+            if (lane + 64 * r == p) st[r] = k;
         }
-        /* the next pivot column first, then its argmax */
+        /* every row as a non-pivot row (the pivot row is rewritten below):
+         * the next pivot column first, then its argmax (the pivot row is
+         * marked used), then the other columns */
         int pn = 0;
         if (c + 1 < NB) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const double nv = v[r][c + 1] - pc[r] * prw[c + 1];
-                v[r][c + 1] = isp[r] ? prw[c + 1] : nv;
-            }
+            for (int r = 0; r < R; ++r) v[r][c + 1] -= pc[r] * prw[c + 1];
             if (c + 1 < nc) pn = gj_col_argmax<R, NB>(v, st, c + 1);
         }
 #pragma unroll
@@ -613,8 +610,13 @@ __device__ __forceinline__ void gj_factor(const GjView& g, int n, int k0, double
 #pragma unroll
             for (int c2 = 0; c2 < NB; ++c2) {
                 if (c2 == c + 1) continue;
-                const double nv = ((c2 == c) ? 0.0 : v[r][c2]) - pc[r] * prw[c2];
-                v[r][c2] = isp[r] ? prw[c2] : nv;
+                v[r][c2] = ((c2 == c) ? 0.0 : v[r][c2]) - pc[r] * prw[c2];
+            }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (r == pr && lane == pl) {            /* one lane of one slot */
+#pragma unroll
+                for (int c2 = 0; c2 < NB; ++c2) v[r][c2] = prw[c2];
             }
         if (lane == 0) {
             g.piv[k] = p;
