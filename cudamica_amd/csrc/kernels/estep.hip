@@ -389,10 +389,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 /* M = 1: the model posterior v is 1, so the waves of a workgroup (the
  * 64-channel blocks of one range of samples) never exchange anything: no
  * barrier, no LDS, and phase 2 of a sample follows its phase 1 in
- * registers.  One sample per iteration, loads 4 samples ahead; the
- * statistics are summed in FP32 registers over 32 samples, in a second
- * FP32 register set over 1024 samples and in FP64 in the workgroup's slot
- * of the partial buffer.  The log-likelihood sum is sum_t sum_i q_ti (the
+ * registers.  One sample per iteration, loads 3-4 samples ahead; the
+ * statistics are summed in FP32 registers over 32 samples, in FP32 per lane
+ * in LDS over 1024 samples (odd stride: distinct banks) and in FP64 in the
+ * workgroup's slot of the partial buffer.  The log-likelihood sum is sum_t sum_i q_ti (the
  * model constant is added by the reduction); vsum is the sample count.
  * Not used when per-sample L is recorded (ltall). */
 template <int MJ, bool B64>
@@ -422,9 +422,15 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
             nsbmu64[j] = act ? dj[kCst64Nsbmu * ks] : 1.0;
         }
     }
-    float acc[NS], acc2[NS];
+    constexpr int NSP = NS | 1;
+    extern __shared__ float tot2_all[];
+    float* const tot2 = tot2_all + threadIdx.x * NSP;
+    float acc[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) acc[s] = acc2[s] = 0.f;
+    for (int s = 0; s < NS; ++s) {
+        acc[s] = 0.f;
+        tot2[s] = 0.f;
+    }
     float qacc = 0.f;
     double qtot = 0.0;
     double* const base = a.part + (size_t)blockIdx.x * a.pstride;
@@ -436,19 +442,23 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
     const size_t bcol = (size_t)i;
     uint4* const g8 = (uint4*)a.g;
 
-    /* a 4-sample prefetch window, rotated one sample per iteration */
-    float bq0, bq1, bq2, bq3, xq0, xq1, xq2, xq3;
-    double dq0 = 0.0, dq1 = 0.0, dq2 = 0.0, dq3 = 0.0;
-    auto ld = [&](long t, float& bq, double& dq, float& xq) {
+    /* a prefetch window of PF samples, rotated one sample per iteration
+     * (3 with FP64 b, which keeps the kernel within 128 VGPRs) */
+    constexpr int PF = B64 ? 3 : 4;
+    float bq[PF], xq[PF];
+    double dq[PF];
+    auto ld = [&](long t, float& bqv, double& dqv, float& xqv) {
         const bool ok = act && t < t1;
-        if constexpr (B64) dq = ok ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
-        else bq = ok ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
-        xq = ok ? a.x[(size_t)t * a.NP + i] : 0.f;
+        if constexpr (B64) dqv = ok ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
+        else bqv = ok ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
+        xqv = ok ? a.x[(size_t)t * a.NP + i] : 0.f;
     };
-    ld(t0, bq0, dq0, xq0);
-    ld(t0 + 1, bq1, dq1, xq1);
-    ld(t0 + 2, bq2, dq2, xq2);
-    ld(t0 + 3, bq3, dq3, xq3);
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+        bq[p] = 0.f;
+        dq[p] = 0.0;
+        ld(t0 + p, bq[p], dq[p], xq[p]);
+    }
     /* packed bf16 g: pairs of samples shift into gw0..gw3 */
     uint32_t gw0 = 0, gw1 = 0, gw2 = 0, gw3 = 0;
     float gprev = 0.f;
@@ -456,13 +466,16 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
 #pragma unroll 1
     for (long t = t0; t < tend; ++t) {
         const bool valid = t < t1;
-        const float bcur = B64 ? 0.f : bq0;
-        const double dcur = dq0;
-        const float xcur = xq0;
-        bq0 = bq1; bq1 = bq2; bq2 = bq3;
-        dq0 = dq1; dq1 = dq2; dq2 = dq3;
-        xq0 = xq1; xq1 = xq2; xq2 = xq3;
-        ld(t + 4, bq3, dq3, xq3);
+        const float bcur = B64 ? 0.f : bq[0];
+        const double dcur = dq[0];
+        const float xcur = xq[0];
+#pragma unroll
+        for (int p = 0; p + 1 < PF; ++p) {
+            bq[p] = bq[p + 1];
+            dq[p] = dq[p + 1];
+            xq[p] = xq[p + 1];
+        }
+        ld(t + PF, bq[PF - 1], dq[PF - 1], xq[PF - 1]);
         const float bv = B64 ? (float)dcur : bcur;
         float q2[MJ], E[MJ], SY[MJ], YM2[MJ], YR[MJ], LA[MJ];
 #pragma unroll
@@ -527,14 +540,14 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
         }
         if ((k & 31) == 31) {
 #pragma unroll
-            for (int s = 0; s < NS; ++s) { acc2[s] += acc[s]; acc[s] = 0.f; }
+            for (int s = 0; s < NS; ++s) { tot2[s] += acc[s]; acc[s] = 0.f; }
             qtot += (double)qacc;
             qacc = 0.f;
             if ((k & 1023) == 1023) {
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    l3[s] = l3_started ? l3[s] + (double)acc2[s] : (double)acc2[s];
-                    acc2[s] = 0.f;
+                    l3[s] = l3_started ? l3[s] + (double)tot2[s] : (double)tot2[s];
+                    tot2[s] = 0.f;
                 }
                 l3_started = true;
             }
@@ -542,7 +555,7 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        const double v = (double)acc2[s] + (double)acc[s];
+        const double v = (double)tot2[s] + (double)acc[s];
         const double r = l3_started ? l3[s] + v : v;
         l3[s] = act ? r : 0.0;
     }
@@ -570,6 +583,12 @@ int estep_configure(int MJ, size_t lds_bytes) {
         if (r == hipSuccess)                                                                      \
             r = hipFuncSetAttribute((const void*)estep_kernel<J, false>,                         \
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
+        if (r == hipSuccess)                                                                      \
+            r = hipFuncSetAttribute((const void*)estep_m1_kernel<J, true>,                       \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
+        if (r == hipSuccess)                                                                      \
+            r = hipFuncSetAttribute((const void*)estep_m1_kernel<J, false>,                      \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
         break;
     switch (MJ) {
         AMICA_ESTEP_ATTR(1)
@@ -592,15 +611,15 @@ size_t estep_lds_bytes(int MJ, int threads) {
 
 int estep_blocks_per_cu(int MJ, bool b64, int threads, bool m1) {
     int nb = 0;
-    const size_t shm = m1 ? 0 : estep_lds_bytes(MJ, threads);
+    const size_t shm = estep_lds_bytes(MJ, threads);
     hipError_t r = hipErrorInvalidValue;
 #define AMICA_ESTEP_OCC(J)                                                                        \
     case J:                                                                                       \
         if (m1)                                                                                   \
             r = b64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_m1_kernel<J, true>, \
-                                                                   threads, 0)                    \
+                                                                   threads, shm)                  \
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_m1_kernel<J, false>,\
-                                                                   threads, 0);                   \
+                                                                   threads, shm);                 \
         else                                                                                      \
             r = b64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_kernel<J, true>,    \
                                                                    threads, shm)                  \
@@ -631,8 +650,8 @@ int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
 #define AMICA_ESTEP_CASE(J)                                                                   \
     case J:                                                                                   \
         if (m1) {                                                                             \
-            if (b64) hipLaunchKernelGGL((estep_m1_kernel<J, true>), grid, block, 0, st, a);   \
-            else hipLaunchKernelGGL((estep_m1_kernel<J, false>), grid, block, 0, st, a);      \
+            if (b64) hipLaunchKernelGGL((estep_m1_kernel<J, true>), grid, block, shm, st, a); \
+            else hipLaunchKernelGGL((estep_m1_kernel<J, false>), grid, block, shm, st, a);    \
         } else {                                                                              \
             if (b64) hipLaunchKernelGGL((estep_kernel<J, true>), grid, block, shm, st, a);    \
             else hipLaunchKernelGGL((estep_kernel<J, false>), grid, block, shm, st, a);       \
