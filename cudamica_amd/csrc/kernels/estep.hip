@@ -178,14 +178,19 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     long gbase = t0;
     uint4* const g8 = (uint4*)a.g;
 
-    /* the loads of the first tile */
-    float bn[kS];
+    /* the loads of the first tile.  Loads carry no channel check: lanes
+     * i in [n, NP) read the padding columns of b and x, which hold zeros
+     * (the padding rows of W and entries of W c are zero, and x's padding
+     * columns are written as zero), so every load is unconditional and the
+     * wait for a prefetch can sit where its value is first used. */
+    float bn[kS], xn[kS];
     double bn64[B64 ? kS : 1];
 #pragma unroll
     for (int s = 0; s < kS; ++s) {
         const long t = t0 + s;
-        if constexpr (B64) bn64[s] = act ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
-        else bn[s] = act ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
+        if constexpr (B64) bn64[s] = a.b64[(size_t)t * a.ldb + bcol];
+        else bn[s] = a.b[(size_t)t * a.ldb + bcol];
+        xn[s] = a.x[(size_t)t * a.NP + i];
     }
 
     for (long tb = t0; tb < t1; tb += kS) {
@@ -196,18 +201,18 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
         for (int s = 0; s < kS; ++s) {
             if constexpr (B64) { BV64[s] = bn64[s]; BV[s] = (float)bn64[s]; }
             else BV[s] = bn[s];
+            XV[s] = xn[s];
         }
-        /* issue the next tile's b loads and this tile's x loads now; they
-         * complete under this tile's math.  Rows past t1 are read without a
-         * check (the next range's samples, or the kEstepRowPad zero rows
-         * after the last one) and get posterior 0. */
+        /* issue the next tile's b and x loads now; they complete under this
+         * tile's math.  Rows past t1 are read without a check (the next
+         * range's samples, or the kEstepRowPad zero rows after the last
+         * one) and get posterior 0. */
 #pragma unroll
         for (int s = 0; s < kS; ++s) {
             const long tn = tb + kS + s;
-            if constexpr (B64) bn64[s] = act ? a.b64[(size_t)tn * a.ldb + bcol] : 0.0;
-            else bn[s] = act ? a.b[(size_t)tn * a.ldb + bcol] : 0.f;
-            const long t = tb + s;
-            XV[s] = act ? a.x[(size_t)t * a.NP + i] : 0.f;
+            if constexpr (B64) bn64[s] = a.b64[(size_t)tn * a.ldb + bcol];
+            else bn[s] = a.b[(size_t)tn * a.ldb + bcol];
+            xn[s] = a.x[(size_t)tn * a.NP + i];
         }
         /* ---- phase 1: densities, component posteriors, channel sums ---- */
         float QS[kS];
@@ -447,11 +452,15 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
     constexpr int PF = B64 ? 3 : 4;
     float bq[PF], xq[PF];
     double dq[PF];
+    /* Unconditional loads: samples past t1 read row T, the first of the
+     * zero padding rows, and lanes i in [n, NP) the zero padding columns
+     * of b and x (the padding rows of W and entries of W c are zero, and
+     * x's padding columns are written as zero). */
     auto ld = [&](long t, float& bqv, double& dqv, float& xqv) {
-        const bool ok = act && t < t1;
-        if constexpr (B64) dqv = ok ? a.b64[(size_t)t * a.ldb + bcol] : 0.0;
-        else bqv = ok ? a.b[(size_t)t * a.ldb + bcol] : 0.f;
-        xqv = ok ? a.x[(size_t)t * a.NP + i] : 0.f;
+        const size_t r = t < t1 ? (size_t)t : (size_t)a.T;
+        if constexpr (B64) dqv = a.b64[r * a.ldb + bcol];
+        else bqv = a.b[r * a.ldb + bcol];
+        xqv = a.x[r * a.NP + i];
     };
 #pragma unroll
     for (int p = 0; p < PF; ++p) {
