@@ -70,6 +70,15 @@ __device__ __forceinline__ float dpp(float v) {
                                                                  CTRL, ROWS, BANKS, BOUND));
 }
 
+/* max of v and the value a row rotation by R (1, 2, 4, 8) brings in; every
+ * lane of every row has a source, so the DPP move keeps no old value */
+template <int R>
+__device__ __forceinline__ float row_ror_max(float v) {
+    const float o = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v),
+                                                                       0x120 + R, 0xf, 0xf, false));
+    return fmaxf(v, o);
+}
+
 /* Sum over each row of 16 lanes; the row total lands in the row's lane 15
  * (the other lanes hold partial sums).  Row shifts by 1, 2, 4 and 8 with
  * zeros shifted in, each one v_add_f32 with a DPP source. */
@@ -93,10 +102,12 @@ __device__ __forceinline__ float row_sum_lane15(float v) {
 template <bool B64>
 __device__ __forceinline__ constexpr float y_floor() { return B64 ? 1.17549435e-38f : 0x1p-24f; }
 
-/* two floats -> two round-to-nearest-even bf16 in one dword (a low) */
+/* two floats -> two round-to-nearest-even bf16 in one dword (a low): one
+ * v_cvt_pk_bf16_f32 */
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
-    return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)a) |
-           ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)b) << 16);
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2){a, b}, bf2));
 }
 
 template <int MJ, bool B64>
@@ -108,7 +119,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     constexpr int NSJ = kStatsPerJ;
     constexpr int NS = NSJ * MJ + kStatsPerI;
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   /* wave-uniform */
     const int h = w / a.CW;
     const int cb = w - h * a.CW;
     const int i = cb * 64 + lane;
@@ -145,7 +156,11 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             cq2[j] = cj[kCstCq2 * ks];
             rsb[j] = cj[kCstRsb * ks];
         } else {
-            sb[j] = 0.f; nsbmu[j] = 1.f; rm2[j] = 0.f; cq2[j] = 0.f; rsb[j] = 0.f;
+            /* a padding channel: b = 0 gives y = 1, |y|^rho = 1, so
+             * q2 = (0, -1e30, ...) and the component sum is exactly 1: its
+             * log (the channel's share of L) is exactly 0, and g = 0 */
+            sb[j] = 0.f; nsbmu[j] = 1.f; rm2[j] = 0.f; cq2[j] = j == 0 ? AMICA_L2E : -1e30f;
+            rsb[j] = 0.f;
         }
         if constexpr (B64) {
             const double* dj = a.cst64 + ((size_t)h * kNumCst64 * MJ + j) * a.NP + i;
@@ -235,25 +250,25 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
              * units, so 2^q2 needs no max shift unless every component
              * underflows (a sample far from all of them): then, for the
              * whole wave, shift by the per-lane max */
-            float se = 0.f, off = 0.f;
 #pragma unroll
-            for (int j = 0; j < MJ; ++j) {
-                E[s][j] = fast_exp2(q2[j]);
-                se += E[s][j];
-            }
+            for (int j = 0; j < MJ; ++j) E[s][j] = fast_exp2(q2[j]);
+            float se = E[s][0];
+#pragma unroll
+            for (int j = 1; j < MJ; ++j) se += E[s][j];
             if (__builtin_expect(__any(se < 0x1p-100f), 0)) {
-                off = q2[0];
+                float off = q2[0];
 #pragma unroll
                 for (int j = 1; j < MJ; ++j) off = fmaxf(off, q2[j]);
-                se = 0.f;
 #pragma unroll
-                for (int j = 0; j < MJ; ++j) {
-                    E[s][j] = fast_exp2(q2[j] - off);
-                    se += E[s][j];
-                }
+                for (int j = 0; j < MJ; ++j) E[s][j] = fast_exp2(q2[j] - off);
+                se = E[s][0];
+#pragma unroll
+                for (int j = 1; j < MJ; ++j) se += E[s][j];
+                QS[s] = off + fast_log2(se);
+            } else {
+                QS[s] = fast_log2(se);        /* exactly 0 on padding channels */
             }
             RS[s] = fast_rcp(se);
-            QS[s] = act ? off + fast_log2(se) : 0.f;
         }
 #pragma unroll
         for (int s = 0; s < kS; ++s) QS[s] = row_sum_lane15(QS[s]);
@@ -280,21 +295,20 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
              * apart inside the first row of 16 lanes (M <= 8), so row
              * rotations by (1,) 2, 4 and 8 reach all of them */
             float lmax = l;
-            if constexpr (kS == 1) lmax = fmaxf(lmax, dpp<0x121, 0xf, 0xf, false>(lmax));  /* row_ror:1 */
-            lmax = fmaxf(lmax, dpp<0x122, 0xf, 0xf, false>(lmax));      /* row_ror:2 */
-            lmax = fmaxf(lmax, dpp<0x124, 0xf, 0xf, false>(lmax));      /* row_ror:4 */
-            lmax = fmaxf(lmax, dpp<0x128, 0xf, 0xf, false>(lmax));      /* row_ror:8 */
+            if constexpr (kS == 1) lmax = row_ror_max<1>(lmax);
+            lmax = row_ror_max<2>(lmax);
+            lmax = row_ror_max<4>(lmax);
+            lmax = row_ror_max<8>(lmax);
             const float e = (hh < a.M) ? fast_exp2(l - lmax) : 0.f;
             float esum = e;
             if constexpr (kS == 1) esum += dpp<0x121, 0xf, 0xf, false>(esum);
             esum += dpp<0x122, 0xf, 0xf, false>(esum);
             esum += dpp<0x124, 0xf, 0xf, false>(esum);
             esum += dpp<0x128, 0xf, 0xf, false>(esum);
-            const float P2 = lmax + fast_log2(esum);
-            vl = (valid && hh < a.M) ? fast_exp2(l - P2) : 0.f;
+            vl = (valid && hh < a.M) ? e * fast_rcp(esum) : 0.f;
             if (w == 0 && valid && hh < a.M) {
                 vsum_acc += (double)vl;
-                if (hh == 0) ll_acc += (double)P2 * AMICA_LN2;
+                if (hh == 0) ll_acc += (double)(lmax + fast_log2(esum)) * AMICA_LN2;
                 if (a.ltall) a.ltall[(size_t)hh * a.T + t] = l * (float)AMICA_LN2;
             }
         }
@@ -321,12 +335,12 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 aj[5] = fmaf(uyr, YR[s][j], aj[5]);
                 aj[6] = fmaf(uyr, LA[s][j], aj[6]);
             }
-            if (!act) g = 0.f;
+            /* g is 0 on padding channels (rsb = 0) and past t1 (v = 0) */
             const float vb = v * BV[s];
             acc[NSJ * MJ + 0] = fmaf(vb, BV[s], acc[NSJ * MJ + 0]);
             acc[NSJ * MJ + 1] = fmaf(v, XV[s], acc[NSJ * MJ + 1]);
             acc[NSJ * MJ + 2] += g;
-            GV[s] = (t < t1) ? g : 0.f;
+            GV[s] = g;
             if (!a.g_bf16 && t < t1) ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
         }
         if (a.g_bf16) {
