@@ -255,7 +255,9 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             float se = E[s][0];
 #pragma unroll
             for (int j = 1; j < MJ; ++j) se += E[s][j];
-            if (__builtin_expect(__any(se < 0x1p-100f), 0)) {
+            const bool under = __any(se < 0x1p-100f);
+            QS[s] = fast_log2(se);            /* exactly 0 on padding channels */
+            if (__builtin_expect(under, 0)) {
                 float off = q2[0];
 #pragma unroll
                 for (int j = 1; j < MJ; ++j) off = fmaxf(off, q2[j]);
@@ -265,8 +267,6 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 #pragma unroll
                 for (int j = 1; j < MJ; ++j) se += E[s][j];
                 QS[s] = off + fast_log2(se);
-            } else {
-                QS[s] = fast_log2(se);        /* exactly 0 on padding channels */
             }
             RS[s] = fast_rcp(se);
         }
@@ -278,12 +278,13 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
         }
         __syncthreads();
         /* ---- model posteriors: lane hh*kS + s holds model hh, sample s ---- */
+        const int nval = (int)min(t1 - tb, (long)kS);    /* samples of the tile before t1 */
         float vl = 0.f;
         {
             const int hh = lane / kS;
             const int s = lane - hh * kS;
             const long t = tb + s;
-            const bool valid = t < t1;
+            const bool valid = s < nval;
             float l = -3.0e38f;
             if (hh < a.M) {
                 l = dc2[hh];
@@ -299,13 +300,15 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             lmax = row_ror_max<2>(lmax);
             lmax = row_ror_max<4>(lmax);
             lmax = row_ror_max<8>(lmax);
-            const float e = (hh < a.M) ? fast_exp2(l - lmax) : 0.f;
+            /* lanes hh >= M hold l = -3e38: e = 0 (the first row's lmax is
+             * a finite model value; lanes past it are never read) */
+            const float e = fast_exp2(l - lmax);
             float esum = e;
             if constexpr (kS == 1) esum += dpp<0x121, 0xf, 0xf, false>(esum);
             esum += dpp<0x122, 0xf, 0xf, false>(esum);
             esum += dpp<0x124, 0xf, 0xf, false>(esum);
             esum += dpp<0x128, 0xf, 0xf, false>(esum);
-            vl = (valid && hh < a.M) ? e * fast_rcp(esum) : 0.f;
+            vl = valid ? e * fast_rcp(esum) : 0.f;
             if (w == 0 && valid && hh < a.M) {
                 vsum_acc += (double)vl;
                 if (hh == 0) ll_acc += (double)(lmax + fast_log2(esum)) * AMICA_LN2;
@@ -341,7 +344,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             acc[NSJ * MJ + 1] = fmaf(v, XV[s], acc[NSJ * MJ + 1]);
             acc[NSJ * MJ + 2] += g;
             GV[s] = g;
-            if (!a.g_bf16 && t < t1) ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
+            if (!a.g_bf16 && s < nval) ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
         }
         if (a.g_bf16) {
             bool word = true;                      /* a dword (2 samples) complete */

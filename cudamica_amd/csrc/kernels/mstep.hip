@@ -32,6 +32,7 @@ namespace amica {
 namespace {
 
 constexpr int kBlk = 256;
+constexpr int kInvMax = 1024;     /* largest n of the inverse and the M-step */
 
 /* canonical statistics offsets within one model block */
 struct COff {
@@ -183,6 +184,7 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
  * are split with 32-bit division, here and in the kernels below.) */
 __global__ __launch_bounds__(kBlk) void mstep_a_kernel(MstepArgs a) {
     __shared__ double sh[16];
+    __shared__ double ks[kInvMax];          /* kappa_i sigma2_i */
     const int h = blockIdx.x, n = a.n, m = a.m;
     const long nm = (long)n * m;
     const COff o(n, m);
@@ -195,22 +197,18 @@ __global__ __launch_bounds__(kBlk) void mstep_a_kernel(MstepArgs a) {
     double* rho = a.rho + (size_t)h * nm;
     double* sig = a.sigma2 + (size_t)h * n;
     double* cnw = a.cnew + (size_t)h * n;
-    double* phs = a.phis + (size_t)h * n * n;
     double* kap = a.kappa + (size_t)h * n;
     double* lam = a.lambda + (size_t)h * n;
-    double* B = a.B + (size_t)h * n * n;
 
     if (threadIdx.x == 0 && a.update_gm) a.gm[h] = vsum / (double)a.T;
     if (a.update_alpha)
         for (long q = threadIdx.x; q < nm; q += blockDim.x) alpha[q] = st[o.usum + q] / vsum;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        sig[i] = st[o.s2 + i] / vsum;
-        cnw[i] = st[o.cnew + i] / vsum;
-    }
-    for (long q = threadIdx.x; q < (long)n * n; q += blockDim.x) phs[q] = st[o.phi + q] / vsum;
     __syncthreads();
     /* updatekappalambda with the new alpha and the old mu */
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double sg = st[o.s2 + i] / vsum;
+        sig[i] = sg;
+        cnw[i] = st[o.cnew + i] / vsum;
         double kk = 0.0, ll = 0.0;
         for (int j = 0; j < m; ++j) {
             const long q = (long)j * n + i;
@@ -223,6 +221,7 @@ __global__ __launch_bounds__(kBlk) void mstep_a_kernel(MstepArgs a) {
         }
         kap[i] = kk;
         lam[i] = ll;
+        ks[i] = kk * sg;
     }
     __syncthreads();
     for (long q = threadIdx.x; q < nm; q += blockDim.x) {
@@ -237,31 +236,18 @@ __global__ __launch_bounds__(kBlk) void mstep_a_kernel(MstepArgs a) {
             rho[q] = r;
         }
     }
-    /* getbflag: D_ik = kappa_i sigma2_i kappa_k sigma2_k <= 1, diagonal included */
-    double cnt = 0.0;
-    for (long q = threadIdx.x; q < (long)n * n; q += blockDim.x) {
-        const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
-        const double D = (kap[i] * sig[i]) * (kap[k] * sig[k]);
-        if (D <= 1.0) cnt += 1.0;
-    }
-    const double bsum = block_sum(cnt, sh);
-    const int bflag = (int)bsum;
-    const int newt = (bflag == 0 && a.do_newton == 1 && k_it >= a.newt_start_iter) ? 1 : 0;
-    for (long q = threadIdx.x; q < (long)n * n; q += blockDim.x) {
-        const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
-        double bv;
-        if (newt) {
-            if (i == k) {
-                bv = (1.0 - phs[q]) / lam[i];                       /* updateBDiagonal */
-            } else {
-                const double D = (kap[i] * sig[i]) * (kap[k] * sig[k]);
-                const double nom = -sig[i] * kap[k];
-                bv = (nom * -phs[q] + -phs[k + (long)i * n]) / (D - 1.0);   /* getB */
-            }
-        } else {
-            bv = (i == k ? 1.0 : 0.0) - phs[q];                     /* geteyemphi */
+    /* getbflag: D_ik = kappa_i sigma2_i kappa_k sigma2_k <= 1, diagonal
+     * included, counted over all (i, k) from the block's copy of ks; part of
+     * the A update, so neither it nor a Newton step happens without one */
+    int bflag = 0, newt = 0;
+    if (a.update_A) {
+        double cnt = 0.0;
+        for (long q = threadIdx.x; q < (long)n * n; q += blockDim.x) {
+            const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
+            if (ks[i] * ks[k] <= 1.0) cnt += 1.0;
         }
-        B[q] = bv;
+        bflag = (int)block_sum(cnt, sh);
+        newt = (bflag == 0 && a.do_newton == 1 && k_it >= a.newt_start_iter) ? 1 : 0;
     }
     if (threadIdx.x == 0) {
         a.bflag[h] = bflag;
@@ -272,6 +258,38 @@ __global__ __launch_bounds__(kBlk) void mstep_a_kernel(MstepArgs a) {
             a.log_vsum[(size_t)(k_it - 1) * a.M + h] = vsum;
         }
     }
+}
+
+/* phi_h = Phi_h / vsum_h and the direction B_h, one thread per (i, k) over
+ * grid (., M): Newton's B (updateBDiagonal / getB) where model h takes a
+ * Newton step, I - phi (geteyemphi) otherwise. */
+__global__ __launch_bounds__(256) void mstep_bmat_kernel(MstepArgs a) {
+    const int h = blockIdx.y, n = a.n;
+    const long nn = (long)n * n;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nn) return;
+    const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
+    const COff o(n, a.m);
+    const double* st = a.stats + (size_t)h * a.stride;
+    const double vsum = st[0];
+    const double ph = st[o.phi + q] / vsum;
+    a.phis[(size_t)h * nn + q] = ph;
+    double bv;
+    if (a.newton[h]) {
+        const double* sig = a.sigma2 + (size_t)h * n;
+        const double* kap = a.kappa + (size_t)h * n;
+        if (i == k) {
+            bv = (1.0 - ph) / a.lambda[(size_t)h * n + i];          /* updateBDiagonal */
+        } else {
+            const double D = (kap[i] * sig[i]) * (kap[k] * sig[k]);
+            const double nom = -sig[i] * kap[k];
+            const double pt = st[o.phi + k + (long)i * n] / vsum;
+            bv = (nom * -ph + -pt) / (D - 1.0);                    /* getB */
+        }
+    } else {
+        bv = (i == k ? 1.0 : 0.0) - ph;                             /* geteyemphi */
+    }
+    a.B[(size_t)h * nn + q] = bv;
 }
 
 /* The scalar part: LL, the decrease rule and the lrate schedule. */
@@ -309,63 +327,100 @@ __global__ void mstep_b_kernel(MstepArgs a) {
     ds->iter = k_it;
 }
 
-/* A update, reparameterisation and c <- cnew for model blockIdx.x. */
-/* AB_h = A_h B_h for every model, one thread per output element (the
- * threads of a wave share the column of B and read consecutive rows of A). */
-__global__ __launch_bounds__(256) void ab_kernel(MstepArgs a) {
-    const int h = blockIdx.y, n = a.n;
+/* AB_h = A_h B_h for every model (column-major n x n, FP64): 16 x 16
+ * output tiles per workgroup over grid (n/16, n/16, M), the l sum through
+ * 16-wide LDS tiles of A and B, one FMA per l in increasing l. */
+constexpr int kAbTile = 16;
+
+__global__ __launch_bounds__(kAbTile * kAbTile) void ab_kernel(MstepArgs a) {
+    __shared__ double As[kAbTile][kAbTile + 1];   /* [l][i] */
+    __shared__ double Bs[kAbTile][kAbTile + 1];   /* [k][l] */
+    const int h = blockIdx.z, n = a.n;
     const long nn = (long)n * n;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (!a.update_A || q >= nn) return;
-    const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
+    const int ti = threadIdx.x % kAbTile, tk = threadIdx.x / kAbTile;
+    const int i = blockIdx.x * kAbTile + ti, k = blockIdx.y * kAbTile + tk;
     const double* A = a.A + (size_t)h * nn;
-    const double* B = a.B + (size_t)h * nn + (long)k * n;
+    const double* B = a.B + (size_t)h * nn;
     double s = 0.0;
-    for (int l = 0; l < n; ++l) s += A[i + (long)l * n] * B[l];
-    a.work[8 + (size_t)h * nn + q] = s;
+    for (int l0 = 0; l0 < n; l0 += kAbTile) {
+        /* A tile: lanes along i (consecutive in memory); B tile: along l */
+        const int ai = blockIdx.x * kAbTile + ti, al = l0 + tk;
+        As[tk][ti] = (ai < n && al < n) ? A[ai + (long)al * n] : 0.0;
+        const int bl = l0 + ti, bk = blockIdx.y * kAbTile + tk;
+        Bs[tk][ti] = (bl < n && bk < n) ? B[bl + (long)bk * n] : 0.0;
+        __syncthreads();
+        const int lend = min(kAbTile, n - l0);
+        for (int l = 0; l < lend; ++l) s += As[l][ti] * Bs[tk][l];
+        __syncthreads();
+    }
+    if (i < n && k < n) a.work[8 + (size_t)h * nn + i + (long)k * n] = s;
 }
 
-/* A += +-lrate A B, reparameterisation, mu shift with the old W, c = cnew. */
-__global__ __launch_bounds__(1024) void mstep_c_kernel(MstepArgs a) {
-    const int h = blockIdx.x, n = a.n, m = a.m;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+/* A += +-lrate A B and the reparameterisation, one wave per column k of
+ * model blockIdx.y (source k): normalizeAsave divides column k by its norm
+ * nu, normalizemubeta scales mu_jk by nu and beta_jk by 1/nu^2, and the mu
+ * shift subtracts (W_old (cnew - c))_k.  c itself is replaced by cnew in
+ * copy_c_kernel afterwards, once every column has read the old c. */
+constexpr int kColWaves = 4;
+constexpr int kColRegs = kInvMax / 64;
+
+__global__ __launch_bounds__(64 * kColWaves) void mstep_col_kernel(MstepArgs a) {
+    const int h = blockIdx.y, n = a.n, m = a.m;
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * kColWaves + (threadIdx.x >> 6);
+    if (!a.update_A || k >= n) return;
     const long nm = (long)n * m;
     const long nn = (long)n * n;
-    double* A = a.A + (size_t)h * nn;
-    double* c = a.c + (size_t)h * n;
-    double* mu = a.mu + (size_t)h * nm;
-    double* beta = a.beta + (size_t)h * nm;
-    const double* cnw = a.cnew + (size_t)h * n;
-    const double* W = a.W + (size_t)h * nn;
-    const double* AB = a.work + 8 + (size_t)h * nn;
-    if (a.update_A) {
-        const double lr = a.ds->lrate;
-        const double f = a.newton[h] ? lr : -lr;
-        for (long q = threadIdx.x; q < nn; q += blockDim.x) A[q] += f * AB[q];
-        __syncthreads();
-        if (a.do_reparm) {
-            /* normalizeAsave + normalizemubeta: one wave per column */
-            for (int k = wv; k < n; k += nw) {
-                double s = 0.0;
-                for (int i = lane; i < n; i += 64) s += A[i + (long)k * n] * A[i + (long)k * n];
-                const double nu = sqrt(wave_sum_f64(s));
-                for (int i = lane; i < n; i += 64) A[i + (long)k * n] /= nu;
-                for (int j = lane; j < m; j += 64) {
-                    mu[(long)j * n + k] *= nu;
-                    beta[(long)j * n + k] = beta[(long)j * n + k] / (nu * nu);
-                }
-            }
-            __syncthreads();
-            /* mu -= W_old (cnew - c) */
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                double d = 0.0;
-                for (int l = 0; l < n; ++l) d += W[i + (long)l * n] * (cnw[l] - c[l]);
-                for (int j = 0; j < m; ++j) mu[(long)j * n + i] -= d;
-            }
-            __syncthreads();
+    double* A = a.A + (size_t)h * nn + (long)k * n;
+    const double* AB = a.work + 8 + (size_t)h * nn + (long)k * n;
+    const double lr = a.ds->lrate;
+    const double f = a.newton[h] ? lr : -lr;
+    double col[kColRegs];
+    double ss = 0.0;
+#pragma unroll
+    for (int r = 0; r < kColRegs; ++r) {
+        const int i = lane + 64 * r;
+        if (i < n) {
+            col[r] = A[i] + f * AB[i];
+            ss += col[r] * col[r];
         }
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = cnw[i];
+    if (!a.do_reparm) {
+#pragma unroll
+        for (int r = 0; r < kColRegs; ++r) {
+            const int i = lane + 64 * r;
+            if (i < n) A[i] = col[r];
+        }
+        return;
+    }
+    const double nu = sqrt(wave_sum_f64(ss));
+#pragma unroll
+    for (int r = 0; r < kColRegs; ++r) {
+        const int i = lane + 64 * r;
+        if (i < n) A[i] = col[r] / nu;
+    }
+    /* (W_old (cnew - c))_k: row k of W, lanes over l */
+    const double* W = a.W + (size_t)h * nn;
+    const double* c = a.c + (size_t)h * n;
+    const double* cnw = a.cnew + (size_t)h * n;
+    double d = 0.0;
+    for (int l = lane; l < n; l += 64) d += W[k + (long)l * n] * (cnw[l] - c[l]);
+    d = wave_sum_f64(d);
+    double* mu = a.mu + (size_t)h * nm;
+    double* beta = a.beta + (size_t)h * nm;
+    for (int j = lane; j < m; j += 64) {
+#pragma clang fp contract(off)
+        const double ms = mu[(long)j * n + k] * nu;      /* rounded, then shifted */
+        mu[(long)j * n + k] = ms - d;
+        beta[(long)j * n + k] = beta[(long)j * n + k] / (nu * nu);
+    }
+}
+
+/* c <- cnew for every model (grid (., M)) */
+__global__ __launch_bounds__(256) void copy_c_kernel(MstepArgs a) {
+    const int h = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n) a.c[(size_t)h * a.n + i] = a.cnew[(size_t)h * a.n + i];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -383,16 +438,14 @@ __device__ __forceinline__ double joined_w(uint16_t hi, uint16_t lo) {
     return (double)bf16_bits_to_f32(hi) + (double)bf16_bits_to_f32(lo);
 }
 
-/* Everything after W = A^-1, for all models at once (grid (., M)): the GEMM
- * copies of W (rows h NP + i, columns k), W~ c with the W~ the unmixing GEMM
- * multiplies by, and the per-(h, j, i) E-step constants.  Flat index per
- * model: [0, n n) W entries, [n n, n n + n) rows of W~ c, then the n m
- * constants. */
+/* After W = A^-1, for all models at once (grid (., M)): the GEMM copies of
+ * W (rows h NP + i, columns k) and the per-(h, j, i) E-step constants.
+ * Flat index per model: [0, n n) W entries, then the n m constants. */
 __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
     const int h = blockIdx.y, n = a.n, m = a.m, NP = a.NP;
     const long nn = (long)n * n, nm = (long)n * m;
     const double* W = a.W + (size_t)h * nn;
-    const long total = nn + n + nm;
+    const long total = nn + nm;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
         if (q < nn) {
             const int i = (int)((unsigned)q % (unsigned)n), k = (int)((unsigned)q / (unsigned)n);
@@ -407,25 +460,8 @@ __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
             } else {
                 a.Wg64[dst] = wv2;
             }
-        } else if (q < nn + n) {
-            const int i = (int)(q - nn);
-            const double* c = a.c + (size_t)h * n;
-            double s = 0.0;
-            for (int k = 0; k < n; ++k) {
-                double wr = W[i + (long)k * n];
-                if (a.prec_bf16) {
-                    /* the W~ the unmixing GEMM applies (W~ = split(W diag(s)) diag(1/s)) */
-                    const double sk = a.xscale ? (double)a.xscale[k] : 1.0;
-                    uint16_t hi, lo;
-                    split_w(wr * sk, hi, lo);
-                    wr = joined_w(hi, lo) / sk;
-                }
-                s += wr * c[k];
-            }
-            if (a.prec_bf16) a.wc[(size_t)h * NP + i] = (float)s;
-            else a.wc64[(size_t)h * NP + i] = s;
         } else {
-            const long e = q - nn - n;
+            const long e = q - nn;
             const int j = (int)((unsigned)e / (unsigned)n), i = (int)((unsigned)e % (unsigned)n);
             const double al = a.alpha[(size_t)h * nm + e], mv = a.mu[(size_t)h * nm + e];
             const double be = a.beta[(size_t)h * nm + e], r = a.rho[(size_t)h * nm + e];
@@ -444,6 +480,33 @@ __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
                 dj[kCst64Nsbmu * ks] = -sb * mv;
             }
         }
+    }
+}
+
+/* W~ c_h, one wave per row i over grid (n/4, M), lanes over k: the W~ the
+ * unmixing GEMM applies (W~ = split(W diag(s)) diag(1/s) on the BF16 and FP8
+ * paths, W itself on the FP32 path). */
+__global__ __launch_bounds__(256) void wc_kernel(MstepArgs a) {
+    const int h = blockIdx.y, n = a.n, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double* W = a.W + (size_t)h * n * n;
+    const double* c = a.c + (size_t)h * n;
+    double s = 0.0;
+    for (int k = lane; k < n; k += 64) {
+        double wr = W[i + (long)k * n];
+        if (a.prec_bf16) {
+            const double sk = a.xscale ? (double)a.xscale[k] : 1.0;
+            uint16_t hi, lo;
+            split_w(wr * sk, hi, lo);
+            wr = joined_w(hi, lo) / sk;
+        }
+        s += wr * c[k];
+    }
+    s = wave_sum_f64(s);
+    if (lane == 0) {
+        if (a.prec_bf16) a.wc[(size_t)h * a.NP + i] = (float)s;
+        else a.wc64[(size_t)h * a.NP + i] = s;
     }
 }
 
@@ -479,7 +542,6 @@ __device__ __forceinline__ void set_ldet(const MstepArgs& a, int h, double lad) 
  * work + 8 + M n n + h n n, then the pivot bookkeeping after all matrices:
  * step[n] (the step that pivoted row i, -1 while unused) and piv[n] (the
  * pivot row of step k) as ints, then |d_k| per step. */
-constexpr int kInvMax = 1024;
 
 struct GjView {
     double* aw;
@@ -835,11 +897,20 @@ int launch_reduce(const double* part, int nparts, size_t pstride, const double* 
 }
 
 int launch_mstep(const MstepArgs& ma, hipStream_t st) {
+    const int n = ma.n;
+    if (n < 1 || n > kInvMax) return -1;
+    const long nn = (long)n * n;
     hipLaunchKernelGGL(mstep_a_kernel, dim3(ma.M), dim3(kBlk), 0, st, ma);
     hipLaunchKernelGGL(mstep_b_kernel, dim3(1), dim3(64), 0, st, ma);
-    const long nn = (long)ma.n * ma.n;
-    hipLaunchKernelGGL(ab_kernel, dim3((unsigned)((nn + 255) / 256), ma.M), dim3(256), 0, st, ma);
-    hipLaunchKernelGGL(mstep_c_kernel, dim3(ma.M), dim3(1024), 0, st, ma);
+    if (ma.update_A) {
+        hipLaunchKernelGGL(mstep_bmat_kernel, dim3((unsigned)((nn + 255) / 256), ma.M), dim3(256), 0,
+                           st, ma);
+        const unsigned nt = (unsigned)((n + kAbTile - 1) / kAbTile);
+        hipLaunchKernelGGL(ab_kernel, dim3(nt, nt, ma.M), dim3(kAbTile * kAbTile), 0, st, ma);
+        hipLaunchKernelGGL(mstep_col_kernel, dim3((unsigned)((n + kColWaves - 1) / kColWaves), ma.M),
+                           dim3(64 * kColWaves), 0, st, ma);
+    }
+    hipLaunchKernelGGL(copy_c_kernel, dim3((unsigned)((n + 255) / 256), ma.M), dim3(256), 0, st, ma);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -859,9 +930,10 @@ int launch_prepare(const MstepArgs& ma, hipStream_t st) {
     else if (n <= 256) launch_gj<4, 16>(ma, st);
     else if (n <= 512) launch_gj<8, 8>(ma, st);
     else launch_gj<16, 4>(ma, st);
-    const long total = (long)ma.n * ma.n + ma.n + (long)ma.n * ma.m;
+    const long total = (long)ma.n * ma.n + (long)ma.n * ma.m;
     const unsigned gx = (unsigned)std::min<long>((total + 255) / 256, 256);
     hipLaunchKernelGGL(prepare_tail_kernel, dim3(gx, ma.M), dim3(256), 0, st, ma);
+    hipLaunchKernelGGL(wc_kernel, dim3((unsigned)((n + 3) / 4), ma.M), dim3(256), 0, st, ma);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -474,6 +474,12 @@ void estep_all(const Model& P, const double* x, long T, double* stats, double* l
 
 struct IterLog { int bflag[64]; int newton[64]; int lldec; };
 
+void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* mu, double* beta,
+                    const double* c, const std::vector<double>& cnew,
+                    const std::vector<double>& sigma2, const std::vector<double>& kappa,
+                    const std::vector<double>& lambda, const std::vector<double>& phi,
+                    const std::vector<double>& W, double* lrate, int iter, IterLog* lg);
+
 void mstep(const orc_config& cfg, int iter, const double* st,
            double* A, double* c, double* alpha, double* mu, double* beta, double* rho,
            double* gm, const std::vector<double>& W, double* lrate, double* lratemax_local,
@@ -543,7 +549,22 @@ void mstep(const orc_config& cfg, int iter, const double* st,
             }
         }
     }
-    if (!cfg.update_A) return;
+    /* the A update (amica.cu:2156-2447) only under update_A; c <- cnew below
+     * runs either way (amica.cu:2450-2462 follows the update_A block) */
+    if (cfg.update_A) mstep_a_update(cfg, st, A, mu, beta, c, cnew, sigma2, kappa, lambda, phi, W,
+                                     lrate, iter, lg);
+    /* c <- cnew (amica.cu:2456-2462) */
+    for (long k = 0; k < (long)n * M; ++k) c[k] = cnew[k];
+}
+
+void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* mu, double* beta,
+                    const double* c, const std::vector<double>& cnew,
+                    const std::vector<double>& sigma2, const std::vector<double>& kappa,
+                    const std::vector<double>& lambda, const std::vector<double>& phi,
+                    const std::vector<double>& W, double* lrate, int iter, IterLog* lg) {
+    const int n = cfg.n, M = cfg.M, m = cfg.m;
+    const long nm = (long)n * m;
+    (void)st;
     /* Newton / natural gradient (amica.cu:2156-2279) */
     std::vector<double> B((size_t)n * n * M);
     int no_newt = 0;
@@ -615,8 +636,6 @@ void mstep(const orc_config& cfg, int iter, const double* st,
                 for (int i = 0; i < n; ++i) mu[h * nm + (long)j * n + i] -= d[i];
         }
     }
-    /* c <- cnew (amica.cu:2456-2462) */
-    for (long k = 0; k < (long)n * M; ++k) c[k] = cnew[k];
 }
 
 }  // namespace

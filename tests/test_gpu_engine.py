@@ -125,16 +125,17 @@ def test_preprocess_parity(built, sph):
     assert np.max(np.abs(wg - xw)) <= 2e-6 * np.max(np.abs(xw))
 
 
-def _run_both(w, T, iters, precision, newt=None, graph=False):
+def _run_both(w, T, iters, precision, newt=None, graph=False, flags=None):
     n, M, m = w.n, w.M, w.m
     x = ca.synth_data(w.spec(T=T))
     st0 = w.init_state()
     xw, S, mean, ldetS = po.preprocess(x, w.do_sphere)
     nsi = w.newt_start_iter if newt is None else newt
-    cfg = po.default_config(n, T, M, m, maxiter=iters, newt_start_iter=nsi)
+    flags = flags or {}
+    cfg = po.default_config(n, T, M, m, maxiter=iters, newt_start_iter=nsi, **flags)
     fin, logs = po.run(cfg, xw, ldetS, st0)
     with ca.Engine(ca.options(n, T, M, m, precision=precision, max_iter=iters,
-                              newt_start_iter=nsi)) as eng:
+                              newt_start_iter=nsi, **flags)) as eng:
         eng.load_raw(x)
         eng.preprocess(w.do_sphere)
         eng.set_state(st0)
@@ -179,7 +180,7 @@ def test_bf16_mu_denom_at_exact_cancellation(built):
     assert mu <= 1e-2
 
 
-def _oracle_envelope(w, T, iters, newt, fin, logs, seeds=(1, 2), eps=6e-8):
+def _oracle_envelope(w, T, iters, newt, fin, logs, seeds=(1, 2), eps=6e-8, flags=None):
     """Conditioning of the EM trajectory itself: the FP64 oracle rerun on its
     whitened input perturbed by float32-rounding-sized noise (once rounded to
     float32, once with relative N(0, eps) noise per seed).  Returns the
@@ -192,7 +193,7 @@ def _oracle_envelope(w, T, iters, newt, fin, logs, seeds=(1, 2), eps=6e-8):
     x = ca.synth_data(w.spec(T=T))
     st0 = w.init_state()
     xw, _, _, ldetS = po.preprocess(x, w.do_sphere)
-    cfg = po.default_config(n, T, M, m, maxiter=iters, newt_start_iter=newt)
+    cfg = po.default_config(n, T, M, m, maxiter=iters, newt_start_iter=newt, **(flags or {}))
     inputs = [xw.astype(np.float32).astype(np.float64)]
     for s in seeds:
         rng = np.random.default_rng(s)
@@ -265,6 +266,29 @@ def test_full_run_fp32_newton(built):
     assert glog["newton"].sum() > 0
     env, wenv = _oracle_envelope(w, 60000, 30, 10, fin, logs)
     _check_trajectory(fin, logs, gst, glog, w.n, w.M, 1e-6, 1e-4, env=env, wenv=wenv)
+
+
+@pytest.mark.parametrize("flags", [
+    dict(do_reparm=0),
+    dict(update_A=0),
+    dict(update_mu=0, update_beta=0, update_rho=0),
+    dict(update_alpha=0, update_gm=0, do_newton=0),
+], ids=["no_reparm", "no_A", "no_mu_beta_rho", "no_alpha_gm_newton"])
+def test_full_run_update_flags(built, flags):
+    """The M-step's switches (config.c keys update_A, update_mu, ...,
+    do_reparm, do_newton): 12 iterations of a 2-model run with Newton from
+    iteration 4, each switch set off against the oracle with the same
+    switches."""
+    w = ca.WORKLOADS["n16_T20k"]
+    T, iters, newt = 20000, 12, 4
+    wf = ca.Workload("n16_M2_flags", n=16, T=T, M=2, m=3, iters=iters, seed=w.seed,
+                     do_sphere=w.do_sphere, newt_start_iter=newt)
+    _say(f"[run fp32 flags] n16 M2 T={T} {flags}")
+    fin, logs, gst, glog = _run_both(wf, T, iters, ca.PREC_FP32, newt=newt, flags=flags)
+    env, wenv = _oracle_envelope(wf, T, iters, newt, fin, logs, flags=flags)
+    _check_trajectory(fin, logs, gst, glog, wf.n, wf.M, 1e-6, 1e-4, env=env, wenv=wenv)
+    if flags.get("update_A", 1) == 0:
+        np.testing.assert_array_equal(gst["A"], wf.init_state()["A"])
 
 
 def test_full_run_bf16_models(built):

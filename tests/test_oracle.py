@@ -107,6 +107,25 @@ def test_em_iterations_match_numpy(n, M, m, iters, newt):
         np.testing.assert_allclose(fnp[k], st2[k], rtol=1e-7, atol=1e-9, err_msg=k)
 
 
+def test_update_A_off_still_moves_c():
+    """With update_A off the reference skips the whole A update (Newton test,
+    lrate growth, reparameterisation, amica.cu:2156-2447) but still copies
+    cnew into c after that block (amica.cu:2450-2462): A stays, c becomes the
+    posterior-weighted data mean of the first E-step."""
+    n, M, m = 4, 2, 3
+    x, st, flat = _tiny(n, 300, M, m, seed=31)
+    xw, S, mean, ldetS = po.preprocess(x, 1)
+    cfg = po.default_config(n, 300, M, m, maxiter=1, update_A=0)
+    fin, logs = po.run(cfg, xw, ldetS, flat)
+    np.testing.assert_array_equal(fin["A"], flat["A"])
+    stats, _ = po.stats_views(po.estep(xw, ldetS, flat["A"], flat["c"], flat["alpha"], flat["mu"],
+                                       flat["beta"], flat["rho"], flat["gm"], M, m), n, m, M)
+    want = np.concatenate([d["cnew"] / d["vsum"] for d in stats])
+    np.testing.assert_allclose(fin["c"], want, rtol=1e-13)
+    assert not np.allclose(fin["c"], flat["c"])
+    assert (logs["newton"] == 0).all() and (logs["bflag"] == 0).all()
+
+
 def test_one_iteration_agrees_to_1e11():
     x, st, flat = _tiny(5, 400, 2, 3, seed=21)
     xw, S, mean, ldetS = po.preprocess(x, 1)
