@@ -738,25 +738,32 @@ __global__ __launch_bounds__(64) void gj_panel_kernel(MstepArgs a) {
 
 /* Applies panel [kp, kp + NB) to every other column, one wave per column
  * (grid (n + 1, M)).  With kn >= 0, block 0 instead updates the columns of
- * the next panel [kn, kn + NB) in registers and factors them. */
+ * the next panel [kn, kn + NB) in registers and factors them.  Every load a
+ * wave needs (its rows of the panel, the pivot-row entries) is issued before
+ * the first FMA: the q loop runs over all NB panel columns, those past the
+ * matrix (a last, partial panel) with multiplier 0. */
 template <int R, int NB>
 __global__ __launch_bounds__(64) void gj_update_kernel(MstepArgs a, int kp, int kn) {
     const int h = blockIdx.y, n = a.n, lane = threadIdx.x;
     const GjView g = gj_view(a, h);
     const int ncp = min(NB, n - kp);
     __shared__ int pv[NB];
-    if (lane < ncp) pv[lane] = g.piv[kp + lane];
+    __shared__ double on[NB][NB];                   /* block 0: pivot rows x next panel */
+    if (lane < NB) pv[lane] = g.piv[kp + min(lane, ncp - 1)];
     __syncthreads();
     bool inP[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
         bool f = false;
-        for (int q = 0; q < ncp; ++q) f |= (pv[q] == i);
+#pragma unroll
+        for (int q = 0; q < NB; ++q) f |= (q < ncp && pv[q] == i);
         inP[r] = f;
     }
     const double* pan = g.aw + (size_t)kp * n;      /* the applied panel's columns */
     if (kn >= 0 && blockIdx.x == 0) {
+        /* the panel's rows and the pivot rows of the next panel through LDS */
+        __shared__ double ps[NB][R * 64];
         const int ncn = min(NB, n - kn);
         double v[R][NB];
 #pragma unroll
@@ -764,22 +771,29 @@ __global__ __launch_bounds__(64) void gj_update_kernel(MstepArgs a, int kp, int 
             const int i = lane + 64 * r;
 #pragma unroll
             for (int c2 = 0; c2 < NB; ++c2)
-                v[r][c2] = (i < n && c2 < ncn && !inP[r]) ? g.aw[i + (size_t)(kn + c2) * n] : 0.0;
+                v[r][c2] = (i < n && c2 < ncn && !inP[r]) ? g.aw[i + (size_t)(kn + min(c2, ncn - 1)) * n] : 0.0;
         }
-        for (int q = 0; q < ncp; ++q) {
-            double pq[R], o[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = lane + 64 * r;
-                pq[r] = i < n ? pan[i + (size_t)q * n] : 0.0;
+                ps[q][i] = (i < n && q < ncp) ? pan[i + (size_t)q * n] : 0.0;
             }
+        for (int e = lane; e < NB * NB; e += 64) {
+            const int q = e / NB, c2 = e - q * NB;
+            on[q][c2] = (q < ncp && c2 < ncn) ? g.aw[pv[q] + (size_t)(kn + c2) * n] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int q = 0; q < NB; ++q) {
+            double pq[R];
 #pragma unroll
-            for (int c2 = 0; c2 < NB; ++c2)
-                o[c2] = c2 < ncn ? g.aw[pv[q] + (size_t)(kn + c2) * n] : 0.0;
+            for (int r = 0; r < R; ++r) pq[r] = ps[q][lane + 64 * r];
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
-                for (int c2 = 0; c2 < NB; ++c2) v[r][c2] += pq[r] * o[c2];
+                for (int c2 = 0; c2 < NB; ++c2) v[r][c2] += pq[r] * on[q][c2];
         }
         gj_factor<R, NB>(g, n, kn, v);
         gj_store<R, NB>(g, n, kn, v);
@@ -787,21 +801,29 @@ __global__ __launch_bounds__(64) void gj_update_kernel(MstepArgs a, int kp, int 
     }
     const int j = (int)blockIdx.x - (kn >= 0 ? 1 : 0);
     if (j < 0 || j >= n || (j >= kp && j < kp + NB) || (kn >= 0 && j >= kn && j < kn + NB)) return;
+    /* this lane's rows of the panel; columns past n read column kp */
+    double pq[NB][R];
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + 64 * r;
+            pq[q][r] = i < n ? pan[i + (size_t)(q < ncp ? q : 0) * n] : 0.0;
+        }
     double* col = g.aw + (size_t)j * n;
+    double o[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) o[q] = q < ncp ? col[pv[q]] : 0.0;
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
         acc[r] = (i < n && !inP[r]) ? col[i] : 0.0;
     }
-    for (int q = 0; q < ncp; ++q) {
-        const double o = col[pv[q]];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = lane + 64 * r;
-            if (i < n) acc[r] += pan[i + (size_t)q * n] * o;
-        }
-    }
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += pq[q][r] * o[q];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
