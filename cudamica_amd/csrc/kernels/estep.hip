@@ -90,6 +90,18 @@ __device__ __forceinline__ float row_sum_lane15(float v) {
     return v;
 }
 
+/* Sum over the wave, landing in lane 63: the row sums (lanes 15, 31, 47,
+ * 63), then each row adds lane 15 of the row before (row_bcast:15), so that
+ * lane 31 holds rows 0-1 and lane 63 rows 2-3, then rows 2-3 add lane 31
+ * (row_bcast:31).  Rows without a source add 0 (bound_ctrl); only lane 63's
+ * result is used. */
+__device__ __forceinline__ float wave_sum_lane63(float v) {
+    v = row_sum_lane15(v);
+    v += dpp<0x142, 0xf, 0xf, true>(v);
+    v += dpp<0x143, 0xf, 0xf, true>(v);
+    return v;
+}
+
 /* Floor of |y| before log2|y|.  FP32 path (y formed in FP64 from FP64 b):
  * the smallest normal float.  Split-BF16 and FP8 paths (y = sb b - sb mu in
  * FP32 from FP32 b): 2^-24, the FP32 spacing at |sb mu| ~ 1.  There b can
@@ -125,9 +137,8 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     const int i = cb * 64 + lane;
     const bool act = i < a.n;
 
-    /* per-row channel sums of one tile, double buffered:
-     * [buf][wave][row of 16 lanes][sample] */
-    __shared__ __align__(16) float Lp[2][kMaxEstepWaves][4][kS];
+    /* per-wave channel sums of one tile, double buffered: [buf][wave][sample] */
+    __shared__ __align__(16) float Lp[2][kMaxEstepWaves][kS];
     __shared__ float dc2[AMICA_MAX_M];
     /* per-lane FP32 level-2 totals, [blockDim.x][NSP] (odd stride: the
      * lanes of a wave hit distinct banks) */
@@ -185,7 +196,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     const size_t bcol = (size_t)h * a.NP + i;
     int buf = 0, tiles = 0, flushes = 0;
     /* bf16 g, 8 samples packed per column: [t/8][ldb][8].  The words of the
-     * current 8-sample group (2 samples each) shift through gw0..gw3. */
+     * current 8-sample group (2 samples each) are gw0..gw3, word gq next. */
     uint32_t gw0 = 0, gw1 = 0, gw2 = 0, gw3 = 0;
     int gq = 0;
     float gpend = 0.f;       /* one sample per tile: the first sample of a dword */
@@ -271,10 +282,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             RS[s] = fast_rcp(se);
         }
 #pragma unroll
-        for (int s = 0; s < kS; ++s) QS[s] = row_sum_lane15(QS[s]);
-        if ((lane & 15) == 15) {
+        for (int s = 0; s < kS; ++s) QS[s] = wave_sum_lane63(QS[s]);
+        if (lane == 63) {
 #pragma unroll
-            for (int s = 0; s < kS; ++s) Lp[buf][w][lane >> 4][s] = QS[s];
+            for (int s = 0; s < kS; ++s) Lp[buf][w][s] = QS[s];
         }
         __syncthreads();
         /* ---- model posteriors: lane hh*kS + s holds model hh, sample s ---- */
@@ -288,9 +299,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             float l = -3.0e38f;
             if (hh < a.M) {
                 l = dc2[hh];
-                for (int c = 0; c < a.CW; ++c)
-#pragma unroll
-                    for (int row = 0; row < 4; ++row) l += Lp[buf][hh * a.CW + c][row][s];
+                for (int c = 0; c < a.CW; ++c) l += Lp[buf][hh * a.CW + c][s];
             }
             /* logsumexp over the models: the lanes of one sample are kS
              * apart inside the first row of 16 lanes (M <= 8), so row
@@ -357,7 +366,12 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 word = (godd ^= 1) == 0;
             }
             if (word) {
-                gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = pack_bf16x2(g0, g1);
+                const uint32_t wd = pack_bf16x2(g0, g1);
+                /* gq is wave-uniform: a scalar branch picks the word */
+                if (gq == 0) gw0 = wd;
+                else if (gq == 1) gw1 = wd;
+                else if (gq == 2) gw2 = wd;
+                else gw3 = wd;
                 if (++gq == 4) {
                     g8[(size_t)(gbase >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
                     gq = 0;
@@ -386,8 +400,18 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
         }
     }
     if (a.g_bf16 && (gq != 0 || godd != 0)) {   /* a partial group at the end of the recording */
-        if (godd) { gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = pack_bf16x2(gpend, 0.f); ++gq; }
-        while (gq < 4) { gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = 0u; ++gq; }
+        if (godd) {
+            const uint32_t wd = pack_bf16x2(gpend, 0.f);
+            if (gq == 0) gw0 = wd;
+            else if (gq == 1) gw1 = wd;
+            else if (gq == 2) gw2 = wd;
+            else gw3 = wd;
+            ++gq;
+        }
+        /* the words past the last sample are zero */
+        if (gq < 2) gw1 = 0u;
+        if (gq < 3) gw2 = 0u;
+        if (gq < 4) gw3 = 0u;
         g8[(size_t)(gbase >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
     }
 #pragma unroll
@@ -485,7 +509,7 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
         dq[p] = 0.0;
         ld(t0 + p, bq[p], dq[p], xq[p]);
     }
-    /* packed bf16 g: pairs of samples shift into gw0..gw3 */
+    /* packed bf16 g: word (k / 2) mod 4 of the 8-sample group */
     uint32_t gw0 = 0, gw1 = 0, gw2 = 0, gw3 = 0;
     float gprev = 0.f;
     const long tend = t0 + (t1 - t0 + G - 1) / G * G;   /* whole 8-sample groups */
@@ -557,7 +581,12 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
         const int k = (int)(t - t0);                  /* t0 is a multiple of 8 */
         if (a.g_bf16) {
             if (k & 1) {
-                gw0 = gw1; gw1 = gw2; gw2 = gw3; gw3 = pack_bf16x2(gprev, g);
+                const uint32_t wd = pack_bf16x2(gprev, g);
+                const int q = (k >> 1) & 3;            /* wave-uniform */
+                if (q == 0) gw0 = wd;
+                else if (q == 1) gw1 = wd;
+                else if (q == 2) gw2 = wd;
+                else gw3 = wd;
                 if ((k & 7) == 7) g8[(size_t)((t - 7) >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
             }
             gprev = g;
