@@ -156,6 +156,11 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 
     float sb[MJ], nsbmu[MJ], rm2[MJ], cq2[MJ], rsb[MJ];
     double sb64[B64 ? MJ : 1], nsbmu64[B64 ? MJ : 1];
+    /* FP32 path, m = 3: the FP64 constants live in LDS ([j][2][thread],
+     * conflict-free) and are read once per tile (the barrier orders them),
+     * which keeps the kernel within 128 VGPRs: 4 waves per SIMD */
+    constexpr bool kC64Lds = B64 && MJ == 3;
+    __shared__ double c64s[kC64Lds ? 2 * MJ * 512 : 1];
 #pragma unroll
     for (int j = 0; j < MJ; ++j) {
         const float* cj = a.cst + ((size_t)h * kNumCst * MJ + j) * a.NP + i;
@@ -177,6 +182,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             const double* dj = a.cst64 + ((size_t)h * kNumCst64 * MJ + j) * a.NP + i;
             sb64[j] = act ? dj[kCst64Sb * ks] : 0.0;
             nsbmu64[j] = act ? dj[kCst64Nsbmu * ks] : 1.0;
+            if constexpr (kC64Lds) {
+                c64s[(2 * j) * 512 + threadIdx.x] = sb64[j];
+                c64s[(2 * j + 1) * 512 + threadIdx.x] = nsbmu64[j];
+            }
         }
     }
 #pragma unroll
@@ -248,7 +257,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 #pragma unroll
             for (int j = 0; j < MJ; ++j) {
                 float y;
-                if constexpr (B64) y = (float)fma(sb64[j], BV64[s], nsbmu64[j]);
+                if constexpr (kC64Lds)
+                    y = (float)fma(c64s[(2 * j) * 512 + threadIdx.x], BV64[s],
+                                   c64s[(2 * j + 1) * 512 + threadIdx.x]);
+                else if constexpr (B64) y = (float)fma(sb64[j], BV64[s], nsbmu64[j]);
                 else y = fmaf(sb[j], BV[s], nsbmu[j]);
                 const float la = fast_log2(fmaxf(fabsf(y), y_floor<B64>()));
                 const float ym2 = fast_exp2(rm2[j] * la);
