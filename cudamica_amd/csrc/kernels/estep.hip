@@ -137,8 +137,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     const int i = cb * 64 + lane;
     const bool act = i < a.n;
 
-    /* per-wave channel sums of one tile, double buffered: [buf][wave][sample] */
+    /* per-wave channel sums of one tile, double buffered: [buf][wave][sample],
+     * and the tile's model posteriors [buf][model h * kS + sample] */
     __shared__ __align__(16) float Lp[2][kMaxEstepWaves][kS];
+    __shared__ float Vp[2][AMICA_MAX_M * 2];
     __shared__ float dc2[AMICA_MAX_M];
     /* per-lane FP32 level-2 totals, [blockDim.x][NSP] (odd stride: the
      * lanes of a wave hit distinct banks) */
@@ -300,10 +302,12 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             for (int s = 0; s < kS; ++s) Lp[buf][w][s] = QS[s];
         }
         __syncthreads();
-        /* ---- model posteriors: lane hh*kS + s holds model hh, sample s ---- */
+        /* ---- model posteriors, by wave 0 alone: lane hh*kS + s holds model
+         * hh, sample s; the other waves read them from LDS after a second
+         * barrier ---- */
         const int nval = (int)min(t1 - tb, (long)kS);    /* samples of the tile before t1 */
-        float vl = 0.f;
-        {
+        if (w == 0) {
+            float vl;
             const int hh = lane / kS;
             const int s = lane - hh * kS;
             const long t = tb + s;
@@ -330,18 +334,20 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             esum += dpp<0x124, 0xf, 0xf, false>(esum);
             esum += dpp<0x128, 0xf, 0xf, false>(esum);
             vl = valid ? e * fast_rcp(esum) : 0.f;
-            if (w == 0 && valid && hh < a.M) {
+            if (lane < a.M * kS) Vp[buf][lane] = vl;
+            if (valid && hh < a.M) {
                 vsum_acc += (double)vl;
                 if (hh == 0) ll_acc += (double)(lmax + fast_log2(esum)) * AMICA_LN2;
                 if (a.ltall) a.ltall[(size_t)hh * a.T + t] = l * (float)AMICA_LN2;
             }
         }
+        __syncthreads();
         /* ---- phase 2: responsibilities and statistics ---- */
         float GV[kS];
 #pragma unroll
         for (int s = 0; s < kS; ++s) {
             const long t = tb + s;
-            const float v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vl), h * kS + s));
+            const float v = Vp[buf][h * kS + s];
             const float vrs = v * RS[s];
             float g = 0.f;
 #pragma unroll
