@@ -58,6 +58,7 @@ namespace amica {
 namespace {
 
 constexpr int kS = 2;          /* samples per barrier tile (M = 1 kernel) */
+constexpr int kBufferRsrcWord3 = 0x00020000;   /* raw 32-bit buffer, gfx9-family format */
 constexpr int kFlush2 = 32;    /* register flushes summed in FP32 LDS */
 
 __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
@@ -87,18 +88,6 @@ __device__ __forceinline__ float row_sum_lane15(float v) {
     v += dpp<0x112, 0xf, 0xf, true>(v);
     v += dpp<0x114, 0xf, 0xf, true>(v);
     v += dpp<0x118, 0xf, 0xf, true>(v);
-    return v;
-}
-
-/* Sum over the wave, landing in lane 63: the row sums (lanes 15, 31, 47,
- * 63), then each row adds lane 15 of the row before (row_bcast:15), so that
- * lane 31 holds rows 0-1 and lane 63 rows 2-3, then rows 2-3 add lane 31
- * (row_bcast:31).  Rows without a source add 0 (bound_ctrl); only lane 63's
- * result is used. */
-__device__ __forceinline__ float wave_sum_lane63(float v) {
-    v = row_sum_lane15(v);
-    v += dpp<0x142, 0xf, 0xf, true>(v);
-    v += dpp<0x143, 0xf, 0xf, true>(v);
     return v;
 }
 
@@ -137,9 +126,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     const int i = cb * 64 + lane;
     const bool act = i < a.n;
 
-    /* per-wave channel sums of one tile, double buffered: [buf][wave][sample],
-     * and the tile's model posteriors [buf][model h * kS + sample] */
-    __shared__ __align__(16) float Lp[2][kMaxEstepWaves][kS];
+    /* per-row channel sums of one tile, double buffered:
+     * [buf][wave][row of 16 lanes][sample], and the tile's model posteriors
+     * [buf][model h * kS + sample] */
+    __shared__ __align__(16) float Lp[2][kMaxEstepWaves][4][kS];
     __shared__ float Vp[2][AMICA_MAX_M * 2];
     __shared__ float dc2[AMICA_MAX_M];
     /* per-lane FP32 level-2 totals, [blockDim.x][NSP] (odd stride: the
@@ -222,13 +212,27 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
      * wait for a prefetch can sit where its value is first used. */
     float bn[kS], xn[kS];
     double bn64[B64 ? kS : 1];
+    /* b and x through buffer resources based at row t0: the lane's byte
+     * offset is fixed, the row's a scalar (launch_estep bounds it below
+     * 2^31), so a load costs no address arithmetic on the vector ALU */
+    constexpr int kEb = B64 ? 8 : 4;
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)((const char*)(B64 ? (const void*)a.b64 : (const void*)a.b) + (size_t)t0 * a.ldb * kEb),
+        (short)0, (int)0xffffffff, kBufferRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.x + (size_t)t0 * a.NP), (short)0, (int)0xffffffff, kBufferRsrcWord3);
+    const int vob = (int)bcol * kEb, vox = i * 4;
+    const int browb = a.ldb * kEb, xrowb = a.NP * 4;
+    auto load_tile = [&](long t, int s) {
+        const int r = (int)(t - t0);
+        if constexpr (B64)
+            bn64[s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rb, vob, r * browb, 0));
+        else
+            bn[s] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rb, vob, r * browb, 0));
+        xn[s] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, vox, r * xrowb, 0));
+    };
 #pragma unroll
-    for (int s = 0; s < kS; ++s) {
-        const long t = t0 + s;
-        if constexpr (B64) bn64[s] = a.b64[(size_t)t * a.ldb + bcol];
-        else bn[s] = a.b[(size_t)t * a.ldb + bcol];
-        xn[s] = a.x[(size_t)t * a.NP + i];
-    }
+    for (int s = 0; s < kS; ++s) load_tile(t0 + s, s);
 
     for (long tb = t0; tb < t1; tb += kS) {
         float E[kS][MJ], SY[kS][MJ], YM2[kS][MJ], YR[kS][MJ], LA[kS][MJ];
@@ -245,12 +249,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
          * range's samples, or the kEstepRowPad zero rows after the last
          * one) and get posterior 0. */
 #pragma unroll
-        for (int s = 0; s < kS; ++s) {
-            const long tn = tb + kS + s;
-            if constexpr (B64) bn64[s] = a.b64[(size_t)tn * a.ldb + bcol];
-            else bn[s] = a.b[(size_t)tn * a.ldb + bcol];
-            xn[s] = a.x[(size_t)tn * a.NP + i];
-        }
+        for (int s = 0; s < kS; ++s) load_tile(tb + kS + s, s);
         /* ---- phase 1: densities, component posteriors, channel sums ---- */
         float QS[kS];
 #pragma unroll
@@ -296,10 +295,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             RS[s] = fast_rcp(se);
         }
 #pragma unroll
-        for (int s = 0; s < kS; ++s) QS[s] = wave_sum_lane63(QS[s]);
-        if (lane == 63) {
+        for (int s = 0; s < kS; ++s) QS[s] = row_sum_lane15(QS[s]);
+        if ((lane & 15) == 15) {
 #pragma unroll
-            for (int s = 0; s < kS; ++s) Lp[buf][w][s] = QS[s];
+            for (int s = 0; s < kS; ++s) Lp[buf][w][lane >> 4][s] = QS[s];
         }
         __syncthreads();
         /* ---- model posteriors, by wave 0 alone: lane hh*kS + s holds model
@@ -315,7 +314,9 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             float l = -3.0e38f;
             if (hh < a.M) {
                 l = dc2[hh];
-                for (int c = 0; c < a.CW; ++c) l += Lp[buf][hh * a.CW + c][s];
+                for (int c = 0; c < a.CW; ++c)
+#pragma unroll
+                    for (int row = 0; row < 4; ++row) l += Lp[buf][hh * a.CW + c][row][s];
             }
             /* logsumexp over the models: the lanes of one sample are kS
              * apart inside the first row of 16 lanes (M <= 8), so row
@@ -511,15 +512,26 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
     constexpr int PF = B64 ? 3 : 4;
     float bq[PF], xq[PF];
     double dq[PF];
-    /* Unconditional loads: samples past t1 read row T, the first of the
-     * zero padding rows, and lanes i in [n, NP) the zero padding columns
-     * of b and x (the padding rows of W and entries of W c are zero, and
-     * x's padding columns are written as zero). */
+    /* Unconditional loads through buffer resources based at row t0 (scalar
+     * row offsets, bounded by launch_estep): samples past T read row T, the
+     * first of the zero padding rows (a range's samples past t1 below T are
+     * only ever prefetched, never used), and lanes i in [n, NP) the zero
+     * padding columns of b and x (the padding rows of W and entries of W c
+     * are zero, and x's padding columns are written as zero). */
+    constexpr int kEb = B64 ? 8 : 4;
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)((const char*)(B64 ? (const void*)a.b64 : (const void*)a.b) + (size_t)t0 * a.ldb * kEb),
+        (short)0, (int)0xffffffff, kBufferRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.x + (size_t)t0 * a.NP), (short)0, (int)0xffffffff, kBufferRsrcWord3);
+    const int vob = (int)bcol * kEb, vox = i * 4;
     auto ld = [&](long t, float& bqv, double& dqv, float& xqv) {
-        const size_t r = t < t1 ? (size_t)t : (size_t)a.T;
-        if constexpr (B64) dqv = a.b64[r * a.ldb + bcol];
-        else bqv = a.b[r * a.ldb + bcol];
-        xqv = a.x[r * a.NP + i];
+        const int r = (int)(min(t, a.T) - t0);
+        if constexpr (B64)
+            dqv = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rb, vob, r * a.ldb * kEb, 0));
+        else
+            bqv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rb, vob, r * a.ldb * kEb, 0));
+        xqv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, vox, r * a.NP * 4, 0));
     };
 #pragma unroll
     for (int p = 0; p < PF; ++p) {
@@ -716,6 +728,9 @@ int estep_blocks_per_cu(int MJ, bool b64, int threads, bool m1) {
 
 int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
     if (a.M * a.CW > kMaxEstepWaves || a.M > 8 || (a.chunk % 8) != 0) return -1;
+    /* byte offsets of the rows a workgroup reads (its chunk and the next
+     * tile past it) must fit the 31-bit scalar offsets of its buffer loads */
+    if ((a.chunk + 8) * (long)a.ldb * 8 >= (1L << 31)) return -1;
     dim3 grid(nblocks), block(64 * a.M * a.CW);
     const size_t shm = estep_lds_bytes(MJ, (int)block.x);
     const bool b64 = a.b64 != nullptr;
