@@ -20,7 +20,7 @@
  * Layout: b and g are [T][ldb] with column h*NP + i; x is [T][NP]; NP is n
  * rounded up to 64.  Lane = channel, wave = (model h, 64-channel block cb);
  * a workgroup of M*CW waves owns a contiguous range of samples, so the
- * cross-channel sum (wave butterfly + LDS) and the cross-model logsumexp
+ * cross-channel sum (DPP row sums + LDS) and the cross-model logsumexp
  * stay on chip.  Per-element transcendentals are cut to three:
  *   la  = log2|y|
  *   ym2 = 2^((rho-2) la) = |y|^(rho-2),  sy = ym2 y = sign(y)|y|^(rho-1),
@@ -39,11 +39,13 @@
  * workgroup's own slot of the partial buffer, which the reduction kernel
  * sums over workgroups in a fixed order (no atomics, run-to-run identical).
  *
- * Latency: the b (and x) loads of tile k+1 are issued before tile k's math,
- * the per-sample channel sums use DPP row shifts and row broadcasts instead
- * of LDS permutes, one barrier per 4-sample tile exchanges the per-model
- * sums, and the per-lane LDS footprint (FP32 level-2 totals) leaves room for
- * two 512-thread workgroups per CU.
+ * Latency: the b and x loads of tile k+1 are issued before tile k's math
+ * (raw buffer loads, scalar row offsets), the per-sample channel sums use
+ * DPP row shifts instead of LDS permutes, two barriers per tile of 2
+ * samples (1 for m >= 4) publish the per-row channel sums and then the
+ * model posteriors, which one wave evaluates for the workgroup, and the
+ * per-lane LDS footprint (FP32 level-2 totals) leaves room for two
+ * 512-thread workgroups per CU.
  *
  * B64 = true (FP32 path): b is FP64 and y = sqrt(beta)(b - mu) is formed in
  * FP64 before the FP32 density math, so |y| near 0 -- where mu_denom's
@@ -57,7 +59,6 @@ namespace amica {
 
 namespace {
 
-constexpr int kS = 2;          /* samples per barrier tile (M = 1 kernel) */
 constexpr int kBufferRsrcWord3 = 0x00020000;   /* raw 32-bit buffer, gfx9-family format */
 constexpr int kFlush2 = 32;    /* register flushes summed in FP32 LDS */
 
