@@ -128,10 +128,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     const bool act = i < a.n;
 
     /* per-row channel sums of one tile, double buffered:
-     * [buf][wave][row of 16 lanes][sample], and the tile's model posteriors
-     * [buf][model h * kS + sample] */
-    __shared__ __align__(16) float Lp[2][kMaxEstepWaves][4][kS];
-    __shared__ float Vp[2][AMICA_MAX_M * 2];
+     * [buf][wave][sample][row of 16 lanes] (a sample's 4 rows in 16 bytes),
+     * and the tile's model posteriors [buf][model h * kS + sample] */
+    __shared__ __align__(16) float Lp[2][kMaxEstepWaves][kS][4];
+    __shared__ __align__(8) float Vp[2][AMICA_MAX_M * 2];
     __shared__ float dc2[AMICA_MAX_M];
     /* per-lane FP32 level-2 totals, [blockDim.x][NSP] (odd stride: the
      * lanes of a wave hit distinct banks) */
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
         for (int s = 0; s < kS; ++s) QS[s] = row_sum_lane15(QS[s]);
         if ((lane & 15) == 15) {
 #pragma unroll
-            for (int s = 0; s < kS; ++s) Lp[buf][w][lane >> 4][s] = QS[s];
+            for (int s = 0; s < kS; ++s) Lp[buf][w][s][lane >> 4] = QS[s];
         }
         __syncthreads();
         /* ---- model posteriors, by wave 0 alone: lane hh*kS + s holds model
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 l = dc2[hh];
                 for (int c = 0; c < a.CW; ++c)
 #pragma unroll
-                    for (int row = 0; row < 4; ++row) l += Lp[buf][hh * a.CW + c][row][s];
+                    for (int row = 0; row < 4; ++row) l += Lp[buf][hh * a.CW + c][s][row];
             }
             /* logsumexp over the models: the lanes of one sample are kS
              * apart inside the first row of 16 lanes (M <= 8), so row
@@ -345,11 +345,18 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
         }
         __syncthreads();
         /* ---- phase 2: responsibilities and statistics ---- */
-        float GV[kS];
+        float GV[kS], VV[kS];
+        if constexpr (kS == 2) {                      /* both samples in one 8-byte read */
+            const float2 vp = *(const float2*)&Vp[buf][h * 2];
+            VV[0] = vp.x;
+            VV[1] = vp.y;
+        } else {
+            VV[0] = Vp[buf][h];
+        }
 #pragma unroll
         for (int s = 0; s < kS; ++s) {
             const long t = tb + s;
-            const float v = Vp[buf][h * kS + s];
+            const float v = VV[s];
             const float vrs = v * RS[s];
             float g = 0.f;
 #pragma unroll
