@@ -307,6 +307,9 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
          * barrier ---- */
         const int nval = (int)min(t1 - tb, (long)kS);    /* samples of the tile before t1 */
         if (w == 0) {
+            /* the other waves wait for this one at the next barrier: let it
+             * issue ahead of the other workgroup's waves on its SIMD */
+            __builtin_amdgcn_s_setprio(3);
             float vl;
             const int hh = lane / kS;
             const int s = lane - hh * kS;
@@ -342,6 +345,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 if (hh == 0) ll_acc += (double)(lmax + fast_log2(esum)) * AMICA_LN2;
                 if (a.ltall) a.ltall[(size_t)hh * a.T + t] = l * (float)AMICA_LN2;
             }
+            __builtin_amdgcn_s_setprio(0);
         }
         __syncthreads();
         /* ---- phase 2: responsibilities and statistics ---- */
