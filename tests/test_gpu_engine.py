@@ -88,16 +88,19 @@ def test_estep_stats_fp32(built, n, T, M, m):
     _compare_stats(gpu, ora, n, m, M)
 
 
-def test_estep_stats_bf16(built):
+@pytest.mark.parametrize("n,T,M,m", [(128, 20000, 4, 3), (71, 20011, 2, 3), (96, 12345, 2, 4)])
+def test_estep_stats_bf16(built, n, T, M, m):
     """Split-BF16 unmixing (~2^-16 relative b): statistics within 1e-3, the
     LL sum within 1e-5; mu_denom is reported, not bounded (its weight
-    |y|^(rho-2) is singular at y = 0, where b's rounding dominates)."""
-    n, T, M, m = 128, 20000, 4, 3
+    |y|^(rho-2) is singular at y = 0, where b's rounding dominates).  The
+    odd sample counts end in a partial tile and a partial 8-sample group of
+    the packed bf16 g, and n = 71 and 96 leave padding channels in the last
+    64-channel block; m = 4 runs the one-sample tiles."""
     _say(f"[estep bf16] n={n} T={T} M={M} m={m}")
-    x = ca.synth_data(ca.synth_spec(n, T, seed=300, cond_max=0.0))
+    x = ca.synth_data(ca.synth_spec(n, T, seed=300 + n, cond_max=0.0))
     xw, S, mean, ldetS = po.preprocess(x, 1)
     x32 = xw.astype(np.float32).astype(np.float64)
-    st = _random_state(n, M, m, seed=301)
+    st = _random_state(n, M, m, seed=301 + n)
     ora = po.estep(x32, ldetS, st["A"], st["c"], st["alpha"], st["mu"], st["beta"], st["rho"],
                    st["gm"], M, m)
     with ca.Engine(ca.options(n, T, M, m, precision=ca.PREC_BF16)) as eng:
