@@ -38,9 +38,10 @@ enum {
 enum { kRawS2 = 0, kRawCnew = 1, kRawGsum = 2, kStatsPerI = 3 };
 
 constexpr int kMaxEstepWaves = 8;   /* M * ceil(n/64) <= 8 for the fused E-step */
-/* Zero rows after the T sample rows of x, b and b64: the fused E-step
- * prefetches up to 2 * 2 - 1 rows past the end of its range unchecked. */
-constexpr int kEstepRowPad = 4;
+/* Zero rows after the T sample rows of x, b and b64: the fused E-step runs
+ * whole 8-sample groups and prefetches one tile ahead, so it reads up to 9
+ * rows past the end of its range unchecked. */
+constexpr int kEstepRowPad = 16;
 
 /* FP64 per-(h, j, i) constants of the FP64 unmixing path, [M][2][m][NP]:
  * sqrt(beta) and -sqrt(beta) mu, so that y is formed in FP64. */

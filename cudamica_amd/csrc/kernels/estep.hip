@@ -197,13 +197,8 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     const long t1 = min(a.T, t0 + a.chunk);
     const size_t bcol = (size_t)h * a.NP + i;
     int buf = 0, tiles = 0, flushes = 0;
-    /* bf16 g, 8 samples packed per column: [t/8][ldb][8].  The words of the
-     * current 8-sample group (2 samples each) are gw0..gw3, word gq next. */
-    uint32_t gw0 = 0, gw1 = 0, gw2 = 0, gw3 = 0;
-    int gq = 0;
-    float gpend = 0.f;       /* one sample per tile: the first sample of a dword */
-    int godd = 0;
-    long gbase = t0;
+    /* bf16 g, 8 samples packed per column: [t/8][ldb][8], one 16-byte store
+     * per 8-sample group (t0 is a multiple of 8) */
     uint4* const g8 = (uint4*)a.g;
 
     /* the loads of the first tile.  Loads carry no channel check: lanes
@@ -235,215 +230,189 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 #pragma unroll
     for (int s = 0; s < kS; ++s) load_tile(t0 + s, s);
 
-    for (long tb = t0; tb < t1; tb += kS) {
-        float E[kS][MJ], SY[kS][MJ], YM2[kS][MJ], YR[kS][MJ], LA[kS][MJ];
-        float RS[kS], BV[kS], XV[kS];
-        double BV64[B64 ? kS : 1];
+    constexpr int kTPG = 8 / kS;               /* tiles per packed 8-sample group */
+    for (long tg = t0; tg < t1; tg += 8) {
+        /* the group's packed g words (2 samples each); tiles past t1 in the
+         * last group run with posterior 0 and so give g = 0 */
+        uint32_t gw[4] = {0u, 0u, 0u, 0u};
+        float gpend = 0.f;
 #pragma unroll
-        for (int s = 0; s < kS; ++s) {
-            if constexpr (B64) { BV64[s] = bn64[s]; BV[s] = (float)bn64[s]; }
-            else BV[s] = bn[s];
-            XV[s] = xn[s];
-        }
-        /* issue the next tile's b and x loads now; they complete under this
-         * tile's math.  Rows past t1 are read without a check (the next
-         * range's samples, or the kEstepRowPad zero rows after the last
-         * one) and get posterior 0. */
+        for (int sub = 0; sub < kTPG; ++sub) {
+            const long tb = tg + sub * kS;
+            float E[kS][MJ], SY[kS][MJ], YM2[kS][MJ], YR[kS][MJ], LA[kS][MJ];
+            float RS[kS], BV[kS], XV[kS];
+            double BV64[B64 ? kS : 1];
 #pragma unroll
-        for (int s = 0; s < kS; ++s) load_tile(tb + kS + s, s);
-        /* ---- phase 1: densities, component posteriors, channel sums ---- */
-        float QS[kS];
-#pragma unroll
-        for (int s = 0; s < kS; ++s) {
-            float q2[MJ];
-#pragma unroll
-            for (int j = 0; j < MJ; ++j) {
-                float y;
-                if constexpr (kC64Lds)
-                    y = (float)fma(c64s[(2 * j) * 512 + threadIdx.x], BV64[s],
-                                   c64s[(2 * j + 1) * 512 + threadIdx.x]);
-                else if constexpr (B64) y = (float)fma(sb64[j], BV64[s], nsbmu64[j]);
-                else y = fmaf(sb[j], BV[s], nsbmu[j]);
-                const float la = fast_log2(fmaxf(fabsf(y), y_floor<B64>()));
-                const float ym2 = fast_exp2(rm2[j] * la);
-                const float sy = ym2 * y;
-                const float yr = sy * y;
-                q2[j] = fmaf(yr, -AMICA_L2E, cq2[j]);
-                SY[s][j] = sy; YM2[s][j] = ym2; YR[s][j] = yr; LA[s][j] = la;
+            for (int s = 0; s < kS; ++s) {
+                if constexpr (B64) { BV64[s] = bn64[s]; BV[s] = (float)bn64[s]; }
+                else BV[s] = bn[s];
+                XV[s] = xn[s];
             }
-            /* q2 <= log2(alpha sqrt(beta) / (2 Gamma(1 + 1/rho))), a few
-             * units, so 2^q2 needs no max shift unless every component
-             * underflows (a sample far from all of them): then, for the
-             * whole wave, shift by the per-lane max */
+            /* issue the next tile's b and x loads now; they complete under this
+             * tile's math.  Rows past t1 are read without a check (the next
+             * range's samples, or the kEstepRowPad zero rows after the last
+             * one) and get posterior 0. */
 #pragma unroll
-            for (int j = 0; j < MJ; ++j) E[s][j] = fast_exp2(q2[j]);
-            float se = E[s][0];
+            for (int s = 0; s < kS; ++s) load_tile(tb + kS + s, s);
+            /* ---- phase 1: densities, component posteriors, channel sums ---- */
+            float QS[kS];
 #pragma unroll
-            for (int j = 1; j < MJ; ++j) se += E[s][j];
-            const bool under = __any(se < 0x1p-100f);
-            QS[s] = fast_log2(se);            /* exactly 0 on padding channels */
-            if (__builtin_expect(under, 0)) {
-                float off = q2[0];
+            for (int s = 0; s < kS; ++s) {
+                float q2[MJ];
 #pragma unroll
-                for (int j = 1; j < MJ; ++j) off = fmaxf(off, q2[j]);
+                for (int j = 0; j < MJ; ++j) {
+                    float y;
+                    if constexpr (kC64Lds)
+                        y = (float)fma(c64s[(2 * j) * 512 + threadIdx.x], BV64[s],
+                                       c64s[(2 * j + 1) * 512 + threadIdx.x]);
+                    else if constexpr (B64) y = (float)fma(sb64[j], BV64[s], nsbmu64[j]);
+                    else y = fmaf(sb[j], BV[s], nsbmu[j]);
+                    const float la = fast_log2(fmaxf(fabsf(y), y_floor<B64>()));
+                    const float ym2 = fast_exp2(rm2[j] * la);
+                    const float sy = ym2 * y;
+                    const float yr = sy * y;
+                    q2[j] = fmaf(yr, -AMICA_L2E, cq2[j]);
+                    SY[s][j] = sy; YM2[s][j] = ym2; YR[s][j] = yr; LA[s][j] = la;
+                }
+                /* q2 <= log2(alpha sqrt(beta) / (2 Gamma(1 + 1/rho))), a few
+                 * units, so 2^q2 needs no max shift unless every component
+                 * underflows (a sample far from all of them): then, for the
+                 * whole wave, shift by the per-lane max */
 #pragma unroll
-                for (int j = 0; j < MJ; ++j) E[s][j] = fast_exp2(q2[j] - off);
-                se = E[s][0];
+                for (int j = 0; j < MJ; ++j) E[s][j] = fast_exp2(q2[j]);
+                float se = E[s][0];
 #pragma unroll
                 for (int j = 1; j < MJ; ++j) se += E[s][j];
-                QS[s] = off + fast_log2(se);
-            }
-            RS[s] = fast_rcp(se);
-        }
+                const bool under = __any(se < 0x1p-100f);
+                QS[s] = fast_log2(se);            /* exactly 0 on padding channels */
+                if (__builtin_expect(under, 0)) {
+                    float off = q2[0];
 #pragma unroll
-        for (int s = 0; s < kS; ++s) QS[s] = row_sum_lane15(QS[s]);
-        if ((lane & 15) == 15) {
+                    for (int j = 1; j < MJ; ++j) off = fmaxf(off, q2[j]);
 #pragma unroll
-            for (int s = 0; s < kS; ++s) Lp[buf][w][s][lane >> 4] = QS[s];
-        }
-        __syncthreads();
-        /* ---- model posteriors, by wave 0 alone: lane hh*kS + s holds model
-         * hh, sample s; the other waves read them from LDS after a second
-         * barrier ---- */
-        const int nval = (int)min(t1 - tb, (long)kS);    /* samples of the tile before t1 */
-        if (w == 0) {
-            /* the other waves wait for this one at the next barrier: let it
-             * issue ahead of the other workgroup's waves on its SIMD */
-            __builtin_amdgcn_s_setprio(3);
-            float vl;
-            const int hh = lane / kS;
-            const int s = lane - hh * kS;
-            const long t = tb + s;
-            const bool valid = s < nval;
-            float l = -3.0e38f;
-            if (hh < a.M) {
-                l = dc2[hh];
-                for (int c = 0; c < a.CW; ++c)
+                    for (int j = 0; j < MJ; ++j) E[s][j] = fast_exp2(q2[j] - off);
+                    se = E[s][0];
 #pragma unroll
-                    for (int row = 0; row < 4; ++row) l += Lp[buf][hh * a.CW + c][s][row];
-            }
-            /* logsumexp over the models: the lanes of one sample are kS
-             * apart inside the first row of 16 lanes (M <= 8), so row
-             * rotations by (1,) 2, 4 and 8 reach all of them */
-            float lmax = l;
-            if constexpr (kS == 1) lmax = row_ror_max<1>(lmax);
-            lmax = row_ror_max<2>(lmax);
-            lmax = row_ror_max<4>(lmax);
-            lmax = row_ror_max<8>(lmax);
-            /* lanes hh >= M hold l = -3e38: e = 0 (the first row's lmax is
-             * a finite model value; lanes past it are never read) */
-            const float e = fast_exp2(l - lmax);
-            float esum = e;
-            if constexpr (kS == 1) esum += dpp<0x121, 0xf, 0xf, false>(esum);
-            esum += dpp<0x122, 0xf, 0xf, false>(esum);
-            esum += dpp<0x124, 0xf, 0xf, false>(esum);
-            esum += dpp<0x128, 0xf, 0xf, false>(esum);
-            vl = valid ? e * fast_rcp(esum) : 0.f;
-            if (lane < a.M * kS) Vp[buf][lane] = vl;
-            if (valid && hh < a.M) {
-                vsum_acc += (double)vl;
-                if (hh == 0) ll_acc += (double)(lmax + fast_log2(esum)) * AMICA_LN2;
-                if (a.ltall) a.ltall[(size_t)hh * a.T + t] = l * (float)AMICA_LN2;
-            }
-            __builtin_amdgcn_s_setprio(0);
-        }
-        __syncthreads();
-        /* ---- phase 2: responsibilities and statistics ---- */
-        float GV[kS], VV[kS];
-        if constexpr (kS == 2) {                      /* both samples in one 8-byte read */
-            const float2 vp = *(const float2*)&Vp[buf][h * 2];
-            VV[0] = vp.x;
-            VV[1] = vp.y;
-        } else {
-            VV[0] = Vp[buf][h];
-        }
-#pragma unroll
-        for (int s = 0; s < kS; ++s) {
-            const long t = tb + s;
-            const float v = VV[s];
-            const float vrs = v * RS[s];
-            float g = 0.f;
-#pragma unroll
-            for (int j = 0; j < MJ; ++j) {
-                const float u = E[s][j] * vrs;
-                const float uf = u * SY[s][j];
-                g = fmaf(uf, rsb[j], g);
-                float* aj = acc + j * NSJ;
-                aj[0] += u;
-                aj[1] += uf;
-                aj[2] = fmaf(uf, SY[s][j], aj[2]);
-                aj[3] = fmaf(u, YM2[s][j], aj[3]);
-                const float uyr = u * YR[s][j];
-                aj[4] += uyr;
-                aj[5] = fmaf(uyr, YR[s][j], aj[5]);
-                aj[6] = fmaf(uyr, LA[s][j], aj[6]);
-            }
-            /* g is 0 on padding channels (rsb = 0) and past t1 (v = 0) */
-            const float vb = v * BV[s];
-            acc[NSJ * MJ + 0] = fmaf(vb, BV[s], acc[NSJ * MJ + 0]);
-            acc[NSJ * MJ + 1] = fmaf(v, XV[s], acc[NSJ * MJ + 1]);
-            acc[NSJ * MJ + 2] += g;
-            GV[s] = g;
-            if (!a.g_bf16 && s < nval) ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
-        }
-        if (a.g_bf16) {
-            bool word = true;                      /* a dword (2 samples) complete */
-            float g0, g1;
-            if constexpr (kS == 2) {
-                g0 = GV[0]; g1 = GV[1];
-            } else {
-                g0 = gpend; g1 = GV[0];
-                gpend = GV[0];
-                word = (godd ^= 1) == 0;
-            }
-            if (word) {
-                const uint32_t wd = pack_bf16x2(g0, g1);
-                /* gq is wave-uniform: a scalar branch picks the word */
-                if (gq == 0) gw0 = wd;
-                else if (gq == 1) gw1 = wd;
-                else if (gq == 2) gw2 = wd;
-                else gw3 = wd;
-                if (++gq == 4) {
-                    g8[(size_t)(gbase >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
-                    gq = 0;
-                    gbase = tb + kS;
+                    for (int j = 1; j < MJ; ++j) se += E[s][j];
+                    QS[s] = off + fast_log2(se);
                 }
+                RS[s] = fast_rcp(se);
             }
-        }
-        buf ^= 1;
-        if (++tiles == kFlush) {
-            tiles = 0;
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                tot2[s] += acc[s];
-                acc[s] = 0.f;
+            for (int s = 0; s < kS; ++s) QS[s] = row_sum_lane15(QS[s]);
+            if ((lane & 15) == 15) {
+#pragma unroll
+                for (int s = 0; s < kS; ++s) Lp[buf][w][s][lane >> 4] = QS[s];
             }
-            if (++flushes == kFlush2) {
-                flushes = 0;
+            __syncthreads();
+            /* ---- model posteriors, by wave 0 alone: lane hh*kS + s holds model
+             * hh, sample s; the other waves read them from LDS after a second
+             * barrier ---- */
+            const int nval = (int)min(t1 - tb, (long)kS);    /* samples of the tile before t1 */
+            if (w == 0) {
+                /* the other waves wait for this one at the next barrier: let it
+                 * issue ahead of the other workgroup's waves on its SIMD */
+                __builtin_amdgcn_s_setprio(3);
+                float vl;
+                const int hh = lane / kS;
+                const int s = lane - hh * kS;
+                const long t = tb + s;
+                const bool valid = s < nval;
+                float l = -3.0e38f;
+                if (hh < a.M) {
+                    l = dc2[hh];
+                    for (int c = 0; c < a.CW; ++c)
+#pragma unroll
+                        for (int row = 0; row < 4; ++row) l += Lp[buf][hh * a.CW + c][s][row];
+                }
+                /* logsumexp over the models: the lanes of one sample are kS
+                 * apart inside the first row of 16 lanes (M <= 8), so row
+                 * rotations by (1,) 2, 4 and 8 reach all of them */
+                float lmax = l;
+                if constexpr (kS == 1) lmax = row_ror_max<1>(lmax);
+                lmax = row_ror_max<2>(lmax);
+                lmax = row_ror_max<4>(lmax);
+                lmax = row_ror_max<8>(lmax);
+                /* lanes hh >= M hold l = -3e38: e = 0 (the first row's lmax is
+                 * a finite model value; lanes past it are never read) */
+                const float e = fast_exp2(l - lmax);
+                float esum = e;
+                if constexpr (kS == 1) esum += dpp<0x121, 0xf, 0xf, false>(esum);
+                esum += dpp<0x122, 0xf, 0xf, false>(esum);
+                esum += dpp<0x124, 0xf, 0xf, false>(esum);
+                esum += dpp<0x128, 0xf, 0xf, false>(esum);
+                vl = valid ? e * fast_rcp(esum) : 0.f;
+                if (lane < a.M * kS) Vp[buf][lane] = vl;
+                if (valid && hh < a.M) {
+                    vsum_acc += (double)vl;
+                    if (hh == 0) ll_acc += (double)(lmax + fast_log2(esum)) * AMICA_LN2;
+                    if (a.ltall) a.ltall[(size_t)hh * a.T + t] = l * (float)AMICA_LN2;
+                }
+                __builtin_amdgcn_s_setprio(0);
+            }
+            __syncthreads();
+            /* ---- phase 2: responsibilities and statistics ---- */
+            float GV[kS], VV[kS];
+            if constexpr (kS == 2) {                      /* both samples in one 8-byte read */
+                const float2 vp = *(const float2*)&Vp[buf][h * 2];
+                VV[0] = vp.x;
+                VV[1] = vp.y;
+            } else {
+                VV[0] = Vp[buf][h];
+            }
+#pragma unroll
+            for (int s = 0; s < kS; ++s) {
+                const long t = tb + s;
+                const float v = VV[s];
+                const float vrs = v * RS[s];
+                float g = 0.f;
+#pragma unroll
+                for (int j = 0; j < MJ; ++j) {
+                    const float u = E[s][j] * vrs;
+                    const float uf = u * SY[s][j];
+                    g = fmaf(uf, rsb[j], g);
+                    float* aj = acc + j * NSJ;
+                    aj[0] += u;
+                    aj[1] += uf;
+                    aj[2] = fmaf(uf, SY[s][j], aj[2]);
+                    aj[3] = fmaf(u, YM2[s][j], aj[3]);
+                    const float uyr = u * YR[s][j];
+                    aj[4] += uyr;
+                    aj[5] = fmaf(uyr, YR[s][j], aj[5]);
+                    aj[6] = fmaf(uyr, LA[s][j], aj[6]);
+                }
+                /* g is 0 on padding channels (rsb = 0) and past t1 (v = 0) */
+                const float vb = v * BV[s];
+                acc[NSJ * MJ + 0] = fmaf(vb, BV[s], acc[NSJ * MJ + 0]);
+                acc[NSJ * MJ + 1] = fmaf(v, XV[s], acc[NSJ * MJ + 1]);
+                acc[NSJ * MJ + 2] += g;
+                GV[s] = g;
+                if (!a.g_bf16 && s < nval) ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
+            }
+            if constexpr (kS == 2) gw[sub] = pack_bf16x2(GV[0], GV[1]);
+            else if (sub & 1) gw[sub >> 1] = pack_bf16x2(gpend, GV[0]);
+            else gpend = GV[0];
+            buf ^= 1;
+            if (++tiles == kFlush) {
+                tiles = 0;
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    const double v = (double)tot2[s];
-                    l3[s] = l3_started ? l3[s] + v : v;
-                    tot2[s] = 0.f;
+                    tot2[s] += acc[s];
+                    acc[s] = 0.f;
                 }
-                l3_started = true;
+                if (++flushes == kFlush2) {
+                    flushes = 0;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        const double v = (double)tot2[s];
+                        l3[s] = l3_started ? l3[s] + v : v;
+                        tot2[s] = 0.f;
+                    }
+                    l3_started = true;
+                }
             }
         }
-    }
-    if (a.g_bf16 && (gq != 0 || godd != 0)) {   /* a partial group at the end of the recording */
-        if (godd) {
-            const uint32_t wd = pack_bf16x2(gpend, 0.f);
-            if (gq == 0) gw0 = wd;
-            else if (gq == 1) gw1 = wd;
-            else if (gq == 2) gw2 = wd;
-            else gw3 = wd;
-            ++gq;
-        }
-        /* the words past the last sample are zero */
-        if (gq < 2) gw1 = 0u;
-        if (gq < 3) gw2 = 0u;
-        if (gq < 4) gw3 = 0u;
-        g8[(size_t)(gbase >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
+        if (a.g_bf16) g8[(size_t)(tg >> 3) * a.ldb + bcol] = make_uint4(gw[0], gw[1], gw[2], gw[3]);
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -742,7 +711,7 @@ int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
     if (a.M * a.CW > kMaxEstepWaves || a.M > 8 || (a.chunk % 8) != 0) return -1;
     /* byte offsets of the rows a workgroup reads (its chunk and the next
      * tile past it) must fit the 31-bit scalar offsets of its buffer loads */
-    if ((a.chunk + 8) * (long)a.ldb * 8 >= (1L << 31)) return -1;
+    if ((a.chunk + kEstepRowPad) * (long)a.ldb * 8 >= (1L << 31)) return -1;
     dim3 grid(nblocks), block(64 * a.M * a.CW);
     const size_t shm = estep_lds_bytes(MJ, (int)block.x);
     const bool b64 = a.b64 != nullptr;
