@@ -214,7 +214,9 @@ __global__ __launch_bounds__(512) void unmix_lds_kernel(const void* __restrict__
     constexpr int CPR = ROWB / 16;                 /* 16-byte chunks per row */
     constexpr int CHUNKS = PARTS * 16 * CPR;       /* chunks per tile */
     constexpr int NCH = (CHUNKS + 511) / 512;      /* chunks per lane */
-    constexpr int D = kUnmixStages;
+    /* stages in flight: 3 where the W fragments of a split-BF16 NP = 256
+     * wave (2 x 8 pairs) leave no room for a fourth without spilling */
+    constexpr int D = (!Q8 && KS == 8) ? kUnmixStages - 1 : kUnmixStages;
     __shared__ __align__(16) unsigned char lds[2 * SLOT];
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
