@@ -37,6 +37,9 @@ namespace {
 /* The reference's truncated log(2) constant (config.h:25). */
 const double kLog2Ref = 0.693147180559945;
 
+/* Planted mistake of orc_plant (test-only; 0 in every normal use). */
+int g_plant = ORC_PLANT_NONE;
+
 /* ---------------------------------------------------------------------- */
 /* Small dense FP64 linear algebra (column-major)                          */
 /* ---------------------------------------------------------------------- */
@@ -361,8 +364,9 @@ void estep_range(const Model& P, const double* x, long t0, long t1,
             const R* bh = &b[(size_t)h * n];
             a[0] += v[h];
             for (int i = 0; i < n; ++i) {
-                a[o.cnew + i] += (double)(vh * xr[i]);
-                a[o.s2 + i] += (double)(vh * (bh[i] * bh[i]));
+                a[o.cnew + i] += (double)(g_plant == ORC_PLANT_CNEW_NO_V ? xr[i] : vh * xr[i]);
+                a[o.s2 + i] += (double)(g_plant == ORC_PLANT_SIGMA2_FROM_X ? vh * (xr[i] * xr[i])
+                                                                            : vh * (bh[i] * bh[i]));
                 g[i] = 0;
             }
             for (int j = 0; j < m; ++j) {
@@ -373,10 +377,10 @@ void estep_range(const Model& P, const double* x, long t0, long t1,
                     R u = vh * z[q];                                      /* computeu */
                     R fp = rr * F::cs(F::pw(std::fabs(yy), rr - (R)1.0), yy); /* computefp */
                     R ufp = u * fp;
-                    g[i] += sqb[q] * ufp;                                 /* computeg */
+                    g[i] += (g_plant == ORC_PLANT_G_NO_SQRT_BETA ? (R)1 : sqb[q]) * ufp; /* computeg */
                     a[o.usum + k] += (double)u;                           /* getusum */
                     a[o.kn + k] += (double)(ufp * fp);                    /* getdkappanumer */
-                    R lt = fp * yy - (R)1.0;
+                    R lt = fp * yy - (g_plant == ORC_PLANT_LAMBDA_NUMER_PLUS ? (R)-1.0 : (R)1.0);
                     a[o.ln + k] += (double)(u * (lt * lt));               /* getdlambdanumer */
                     a[o.mn + k] += (double)ufp;                           /* getbetamu */
                     if (rr <= (R)2.0) {
@@ -388,7 +392,8 @@ void estep_range(const Model& P, const double* x, long t0, long t1,
                     }
                     if (P.update_rho) {                                   /* getdrhonumer */
                         R pv = F::pw(std::fabs(yy), rr);
-                        if (pv > (R)0) a[o.rn + k] += (double)(u * (pv * F::lg(pv)));
+                        if (pv > (R)0)
+                            a[o.rn + k] += (double)(u * (g_plant == ORC_PLANT_RHO_NUMER_NO_LOG ? pv : pv * F::lg(pv)));
                     }
                 }
             }
@@ -413,8 +418,8 @@ void finalize_stats(int n, int M, int m, const double* beta, const double* rho, 
         double* a = st + h * o.stride;
         for (long k = 0; k < nm; ++k) {
             double be = beta[h * nm + k], r = rho[h * nm + k];
-            a[o.kn + k] *= be;
-            a[o.md + k] *= std::sqrt(be);
+            if (g_plant != ORC_PLANT_KAPPA_NUMER_NO_BETA) a[o.kn + k] *= be;
+            if (g_plant != ORC_PLANT_MU_DENOM_NO_SQRT_BETA) a[o.md + k] *= std::sqrt(be);
             if (r > 2.0) a[o.bd + k] *= r;
             a[o.bn + k] = (r <= 2.0) ? a[o.usum + k] : 0.0;
         }
@@ -478,12 +483,17 @@ void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* 
                     const double* c, const std::vector<double>& cnew,
                     const std::vector<double>& sigma2, const std::vector<double>& kappa,
                     const std::vector<double>& lambda, const std::vector<double>& phi,
-                    const std::vector<double>& W, double* lrate, int iter, IterLog* lg);
+                    const std::vector<double>& W, double* lrate, int iter, IterLog* lg,
+                    orc_mstep_trace* tr);
+
+void copy_out(const std::vector<double>& v, double* dst) {
+    if (dst) std::copy(v.begin(), v.end(), dst);
+}
 
 void mstep(const orc_config& cfg, int iter, const double* st,
            double* A, double* c, double* alpha, double* mu, double* beta, double* rho,
            double* gm, const std::vector<double>& W, double* lrate, double* lratemax_local,
-           int* numdec, const double* ll, IterLog* lg) {
+           int* numdec, const double* ll, IterLog* lg, orc_mstep_trace* tr = nullptr) {
     const int n = cfg.n, M = cfg.M, m = cfg.m;
     const long T = cfg.T;
     const SOff o(n, m);
@@ -492,7 +502,7 @@ void mstep(const orc_config& cfg, int iter, const double* st,
     lg->lldec = 0;
     if (iter > 1 && ll[iter - 1] < ll[iter - 2]) {
         lg->lldec = 1;
-        *lrate = *lrate * cfg.lratefact;
+        if (g_plant != ORC_PLANT_LLDEC_NO_HALVE) *lrate = *lrate * cfg.lratefact;
         *numdec += 1;
         if (*numdec > cfg.maxdec) {
             *lratemax_local *= cfg.lratefact;   /* never used again (B11) */
@@ -500,17 +510,20 @@ void mstep(const orc_config& cfg, int iter, const double* st,
         }
     }
     std::vector<double> vsum(M), sigma2((size_t)n * M), cnew((size_t)n * M), phi((size_t)n * n * M);
+    const std::vector<double> alpha_old(alpha, alpha + (size_t)nm * M);
     for (int h = 0; h < M; ++h) {
         const double* a = st + h * o.stride;
         vsum[h] = a[0];
         if (cfg.update_gm) gm[h] = vsum[h] / (double)T;
         for (long k = 0; k < nm; ++k)
-            if (cfg.update_alpha) alpha[h * nm + k] = a[o.usum + k] / vsum[h];
+            if (cfg.update_alpha)
+                alpha[h * nm + k] = a[o.usum + k] / (g_plant == ORC_PLANT_ALPHA_OVER_T ? (double)T : vsum[h]);
         for (int i = 0; i < n; ++i) {
             sigma2[(size_t)h * n + i] = a[o.s2 + i] / vsum[h];
             cnew[(size_t)h * n + i] = a[o.cnew + i] / vsum[h];
         }
-        for (long k = 0; k < (long)n * n; ++k) phi[(size_t)h * n * n + k] = a[o.phi + k] / vsum[h];
+        for (long k = 0; k < (long)n * n; ++k)
+            phi[(size_t)h * n * n + k] = a[o.phi + k] / (g_plant == ORC_PLANT_PHI_NO_NORM ? 1.0 : vsum[h]);
     }
     /* updatekappalambda: new alpha, old mu (kernels.cu:848-867) */
     std::vector<double> kappa((size_t)n * M, 0.0), lambda((size_t)n * M, 0.0);
@@ -522,16 +535,21 @@ void mstep(const orc_config& cfg, int iter, const double* st,
                 double us = a[o.usum + k];
                 double dk = a[o.kn + k] / us;
                 double al = alpha[h * nm + k];
+                if (g_plant == ORC_PLANT_KAPPA_OLD_ALPHA) al = alpha_old[h * nm + k];
                 double muv = mu[h * nm + k];
                 kappa[(size_t)h * n + i] += dk * al;
-                lambda[(size_t)h * n + i] += (muv * muv * dk + a[o.ln + k] / us) * al;
+                const double mterm = g_plant == ORC_PLANT_LAMBDA_NO_MU ? 0.0 : muv * muv * dk;
+                lambda[(size_t)h * n + i] += (mterm + a[o.ln + k] / us) * al;
             }
     }
     for (int h = 0; h < M; ++h) {
         const double* a = st + h * o.stride;
         for (long k = 0; k < nm; ++k) {
-            if (cfg.update_mu) mu[h * nm + k] += a[o.mn + k] / a[o.md + k];
-            if (cfg.update_beta) beta[h * nm + k] *= a[o.bn + k] / a[o.bd + k];
+            if (cfg.update_mu)
+                mu[h * nm + k] += (g_plant == ORC_PLANT_MU_STEP_SIGN ? -1.0 : 1.0) * a[o.mn + k] / a[o.md + k];
+            if (cfg.update_beta)
+                beta[h * nm + k] *= g_plant == ORC_PLANT_BETA_SQRT_RATIO ? std::sqrt(a[o.bn + k] / a[o.bd + k])
+                                                                         : a[o.bn + k] / a[o.bd + k];
         }
     }
     if (cfg.update_rho && iter >= cfg.rho_start_iter) {
@@ -540,8 +558,8 @@ void mstep(const orc_config& cfg, int iter, const double* st,
             const double* a = st + h * o.stride;
             for (long k = 0; k < nm; ++k) {
                 double r = rho[h * nm + k];
-                double ps = psi_series(1.0 + 1.0 / r);
-                double step = 1.0 - (r / ps) * a[o.rn + k] / a[o.usum + k];
+                double ps = psi_series(g_plant == ORC_PLANT_RHO_PSI_ARG ? 1.0 / r : 1.0 + 1.0 / r);
+                double step = (g_plant == ORC_PLANT_RHO_STEP_SIGN ? -1.0 : 1.0) - (r / ps) * a[o.rn + k] / a[o.usum + k];
                 r = r + cfg.rholrate * step;
                 if (r > cfg.rhomax) r = cfg.rhomax;
                 else if (r < cfg.rhomin) r = cfg.rhomin;
@@ -551,8 +569,12 @@ void mstep(const orc_config& cfg, int iter, const double* st,
     }
     /* the A update (amica.cu:2156-2447) only under update_A; c <- cnew below
      * runs either way (amica.cu:2450-2462 follows the update_A block) */
+    if (tr) {
+        copy_out(kappa, tr->kappa); copy_out(lambda, tr->lambda);
+        copy_out(sigma2, tr->sigma2); copy_out(cnew, tr->cnew); copy_out(phi, tr->phi);
+    }
     if (cfg.update_A) mstep_a_update(cfg, st, A, mu, beta, c, cnew, sigma2, kappa, lambda, phi, W,
-                                     lrate, iter, lg);
+                                     lrate, iter, lg, tr);
     /* c <- cnew (amica.cu:2456-2462) */
     for (long k = 0; k < (long)n * M; ++k) c[k] = cnew[k];
 }
@@ -561,7 +583,8 @@ void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* 
                     const double* c, const std::vector<double>& cnew,
                     const std::vector<double>& sigma2, const std::vector<double>& kappa,
                     const std::vector<double>& lambda, const std::vector<double>& phi,
-                    const std::vector<double>& W, double* lrate, int iter, IterLog* lg) {
+                    const std::vector<double>& W, double* lrate, int iter, IterLog* lg,
+                    orc_mstep_trace* tr) {
     const int n = cfg.n, M = cfg.M, m = cfg.m;
     const long nm = (long)n * m;
     (void)st;
@@ -578,7 +601,7 @@ void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* 
         for (int k = 0; k < n; ++k)
             for (int i = 0; i < n; ++i) {
                 double D = (ka[i] * s2[i]) * (ka[k] * s2[k]);
-                if (D <= 1.0) bflag++;
+                if (D <= 1.0 && !(g_plant == ORC_PLANT_BFLAG_NO_DIAG && i == k)) bflag++;
             }
         lg->bflag[h] = bflag;
         if (bflag > 0) no_newt = 1;
@@ -589,7 +612,10 @@ void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* 
                 for (int i = 0; i < n; ++i) {
                     double D = (ka[i] * s2[i]) * (ka[k] * s2[k]);
                     double nom = -s2[i] * ka[k];
-                    Bh[i + (size_t)k * n] = (nom * -ph[i + (size_t)k * n] + -ph[k + (size_t)i * n]) / (D - 1.0);
+                    const bool sw = g_plant == ORC_PLANT_NEWTON_TRANSPOSE;
+                    const double pik = ph[sw ? k + (size_t)i * n : i + (size_t)k * n];
+                    const double pki = ph[sw ? i + (size_t)k * n : k + (size_t)i * n];
+                    Bh[i + (size_t)k * n] = (nom * -pik + -pki) / (D - 1.0);
                 }
             for (int i = 0; i < n; ++i) Bh[i + (size_t)i * n] = (1.0 - ph[i + (size_t)i * n]) / la[i];
         } else {
@@ -598,16 +624,19 @@ void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* 
                     Bh[i + (size_t)k * n] = (i == k ? 1.0 : 0.0) - ph[i + (size_t)k * n];
         }
     }
+    if (tr) copy_out(B, tr->B);
     /* lrate growth (amica.cu:2281-2285) */
+    const double grow = g_plant == ORC_PLANT_LRATE_GROWTH ? 0.1 : std::min(0.1, *lrate);
     if (iter >= cfg.newt_start_iter && cfg.do_newton && no_newt == 0)
-        *lrate = std::min(cfg.lratemax, *lrate + std::min(0.1, *lrate));
+        *lrate = std::min(cfg.lratemax, *lrate + grow);
     else
-        *lrate = std::min(cfg.lnatrate, *lrate + std::min(0.1, *lrate));
+        *lrate = std::min(cfg.lnatrate, *lrate + grow);
     std::vector<double> AB((size_t)n * n);
     for (int h = 0; h < M; ++h) {
         double* Ah = A + (size_t)h * n * n;
-        matmul(n, Ah, &B[(size_t)h * n * n], AB.data());
-        double f = lg->newton[h] ? *lrate : -*lrate;
+        if (g_plant == ORC_PLANT_AB_ORDER) matmul(n, &B[(size_t)h * n * n], Ah, AB.data());
+        else matmul(n, Ah, &B[(size_t)h * n * n], AB.data());
+        double f = (lg->newton[h] || g_plant == ORC_PLANT_NATGRAD_SIGN) ? *lrate : -*lrate;
         for (long k = 0; k < (long)n * n; ++k) Ah[k] += f * AB[k];
     }
     if (cfg.do_reparm) {
@@ -622,7 +651,7 @@ void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* 
                 for (int j = 0; j < m; ++j) {
                     long q = h * nm + (long)j * n + k;
                     mu[q] *= nu;
-                    beta[q] = beta[q] / (nu * nu);
+                    beta[q] = beta[q] / (g_plant == ORC_PLANT_REPARAM_BETA_NU ? nu : nu * nu);
                 }
             }
         }
@@ -630,8 +659,9 @@ void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* 
             const double* Wh = &W[(size_t)h * n * n];
             std::vector<double> dc(n), d(n, 0.0);
             for (int i = 0; i < n; ++i) dc[i] = cnew[(size_t)h * n + i] - c[(size_t)h * n + i];
+            const bool wt = g_plant == ORC_PLANT_MU_SHIFT_WT;
             for (int k = 0; k < n; ++k)
-                for (int i = 0; i < n; ++i) d[i] += Wh[i + (size_t)k * n] * dc[k];
+                for (int i = 0; i < n; ++i) d[i] += Wh[wt ? k + (size_t)i * n : i + (size_t)k * n] * dc[k];
             for (int j = 0; j < m; ++j)
                 for (int i = 0; i < n; ++i) mu[h * nm + (long)j * n + i] -= d[i];
         }
@@ -641,6 +671,8 @@ void mstep_a_update(const orc_config& cfg, const double* st, double* A, double* 
 }  // namespace
 
 extern "C" {
+
+void orc_plant(int code) { g_plant = code; }
 
 void orc_default_config(orc_config* cfg) {
     std::memset(cfg, 0, sizeof(*cfg));
@@ -818,6 +850,54 @@ int orc_run(const orc_config* cfgp, const double* x_white, double ldetS,
         if (lldec_log) lldec_log[iter - 1] = lg.lldec;
     }
     return 0;
+}
+
+int orc_mstep(const orc_config* cfgp, int iter, const double* stats, const double* W,
+              const double* ll_hist, double* A, double* c, double* alpha, double* mu,
+              double* beta, double* rho, double* gm, double* lrate, int* bflag, int* newton,
+              int* lldec, orc_mstep_trace* tr) {
+    const orc_config& cfg = *cfgp;
+    const int n = cfg.n, M = cfg.M;
+    if (M > 64 || iter < 1) return -1;
+    /* mstep reads ll[iter-1] and ll[iter-2]: place the two values there */
+    std::vector<double> ll((size_t)iter, 0.0);
+    ll[iter - 1] = ll_hist[1];
+    if (iter > 1) ll[iter - 2] = ll_hist[0];
+    std::vector<double> Wv(W, W + (size_t)n * n * M);
+    double lratemax_local = cfg.lratemax;
+    int numdec = 0;
+    IterLog lg;
+    std::memset(&lg, 0, sizeof(lg));
+    mstep(cfg, iter, stats, A, c, alpha, mu, beta, rho, gm, Wv, lrate, &lratemax_local, &numdec,
+          ll.data(), &lg, tr);
+    for (int h = 0; h < M; ++h) {
+        if (bflag) bflag[h] = lg.bflag[h];
+        if (newton) newton[h] = lg.newton[h];
+    }
+    if (lldec) *lldec = lg.lldec;
+    return 0;
+}
+
+int orc_step(const orc_config* cfgp, int iter, const double* x_white, double ldetS,
+             double* A, double* c, double* alpha, double* mu, double* beta, double* rho,
+             double* gm, double* lrate, double ll_prev, double* W, double* ll, int* bflag,
+             int* newton, int* lldec, int precision, int nthreads) {
+    const orc_config& cfg = *cfgp;
+    const int n = cfg.n, M = cfg.M, m = cfg.m;
+    if (M > 64 || iter < 1) return -1;
+    const long stride = SOff(n, m).stride;
+    std::vector<double> st(stride * M + 1);
+    Model P;
+    build_model(P, n, M, m, cfg.update_rho, ldetS, A, c, alpha, mu, beta, rho, gm);
+    if (W) std::copy(P.W.begin(), P.W.end(), W);
+    if (precision == 32) estep_all<float>(P, x_white, cfg.T, st.data(), nullptr, nthreads);
+    else estep_all<double>(P, x_white, cfg.T, st.data(), nullptr, nthreads);
+    finalize_stats(n, M, m, beta, rho, st.data());
+    const double llk = st[stride * M] / ((double)n * (double)cfg.T);
+    if (ll) *ll = llk;
+    const double hist[2] = {ll_prev, llk};
+    return orc_mstep(cfgp, iter, st.data(), P.W.data(), hist, A, c, alpha, mu, beta, rho, gm,
+                     lrate, bflag, newton, lldec, nullptr);
 }
 
 int orc_output_transform(int n, int M, const double* S, const double* mean, double* A, double* c) {

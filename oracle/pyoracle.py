@@ -26,6 +26,13 @@ class OrcConfig(ctypes.Structure):
     ]
 
 
+class MStepTrace(ctypes.Structure):
+    _fields_ = [(k, ctypes.POINTER(ctypes.c_double))
+                for k in ("kappa", "lambda_", "sigma2", "cnew", "phi", "B")]
+
+
+_STATE_KEYS = ("A", "c", "alpha", "mu", "beta", "rho", "gm")
+
 _lib = None
 
 
@@ -48,6 +55,12 @@ def lib():
         L.orc_estep_f32.argtypes = est
         L.orc_run.argtypes = [ctypes.POINTER(OrcConfig), dp, ctypes.c_double, dp, dp, dp, dp, dp,
                               dp, dp, dp, dp, dp, ip, ip, ip, dp, ctypes.c_int, ctypes.c_int]
+        L.orc_mstep.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp,
+                                dp, dp, dp, ip, ip, ip, ctypes.POINTER(MStepTrace)]
+        L.orc_step.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_int, dp, ctypes.c_double, dp, dp, dp,
+                               dp, dp, dp, dp, dp, ctypes.c_double, dp, dp, ip, ip, ip, ctypes.c_int,
+                               ctypes.c_int]
+        L.orc_plant.argtypes = [ctypes.c_int]
         L.orc_pinv.argtypes = [ctypes.c_int, dp, dp]
         L.orc_logabsdet.argtypes = [ctypes.c_int, dp]
         L.orc_logabsdet.restype = ctypes.c_double
@@ -133,6 +146,72 @@ def run(cfg, x_white, ldetS, state, precision=64, nthreads=0, want_ltall=False):
     if want_ltall:
         logs["ltall"] = lt.reshape(M, cfg.T)
     return st, logs
+
+
+def mstep(cfg, it, stats, W, state, lrate, ll_prev=0.0, ll_cur=None, trace=False):
+    """One M-step (orc_mstep) at 1-based iteration `it` from canonical
+    statistics.  ll_cur defaults to the statistics' own log-likelihood.
+    Returns (new state, new lrate, decisions dict, trace dict or None)."""
+    n, M = cfg.n, cfg.M
+    st = {k: np.array(state[k], dtype=np.float64, copy=True).ravel() for k in _STATE_KEYS}
+    stats = np.ascontiguousarray(stats, dtype=np.float64)
+    if ll_cur is None:
+        ll_cur = stats[-1] / (n * cfg.T)
+    hist = np.array([ll_prev, ll_cur], np.float64)
+    lr = np.array([lrate], np.float64)
+    bf = np.zeros(M, np.int32)
+    nw = np.zeros(M, np.int32)
+    dec = np.zeros(1, np.int32)
+    tr = None
+    trp = ctypes.POINTER(MStepTrace)()
+    if trace:
+        tr = dict(kappa=np.zeros(M * n), lambda_=np.zeros(M * n), sigma2=np.zeros(M * n),
+                  cnew=np.zeros(M * n), phi=np.zeros(M * n * n), B=np.zeros(M * n * n))
+        t = MStepTrace(*[_d(tr[k]) for k in ("kappa", "lambda_", "sigma2", "cnew", "phi", "B")])
+        trp = ctypes.pointer(t)
+    Wc = np.ascontiguousarray(W, dtype=np.float64).ravel()
+    rc = lib().orc_mstep(ctypes.byref(cfg), it, _d(stats), _d(Wc), _d(hist),
+                         *[_d(st[k]) for k in _STATE_KEYS], _d(lr), _i(bf), _i(nw), _i(dec), trp)
+    if rc != 0:
+        raise RuntimeError(f"orc_mstep failed: {rc}")
+    return st, float(lr[0]), dict(bflag=bf, newton=nw, lldec=int(dec[0])), tr
+
+
+def step(cfg, it, x_white, ldetS, state, lrate, ll_prev=0.0, precision=64, nthreads=0):
+    """One EM iteration (orc_step) at 1-based iteration `it` from `state`
+    (not modified).  Returns (new state with W of this iteration, LL of this
+    iteration, new lrate, decisions dict)."""
+    n, M = cfg.n, cfg.M
+    st = {k: np.array(state[k], dtype=np.float64, copy=True).ravel() for k in _STATE_KEYS}
+    lr = np.array([lrate], np.float64)
+    W = np.zeros(n * n * M)
+    ll = np.zeros(1)
+    bf = np.zeros(M, np.int32)
+    nw = np.zeros(M, np.int32)
+    dec = np.zeros(1, np.int32)
+    rc = lib().orc_step(ctypes.byref(cfg), it, _d(np.ascontiguousarray(x_white)), ldetS,
+                        *[_d(st[k]) for k in _STATE_KEYS], _d(lr), float(ll_prev), _d(W), _d(ll),
+                        _i(bf), _i(nw), _i(dec), precision, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"orc_step failed: {rc}")
+    st["W"] = W
+    return st, float(ll[0]), float(lr[0]), dict(bflag=bf, newton=nw, lldec=int(dec[0]))
+
+
+# planted-mistake codes of orc_plant (oracle.h), test-only
+PLANTS = {
+    "kappa_numer_no_beta": 1, "lambda_numer_plus": 2, "mu_denom_no_sqrt_beta": 3,
+    "rho_numer_no_log": 4, "sigma2_from_x": 5, "cnew_no_v": 6, "g_no_sqrt_beta": 7,
+    "kappa_old_alpha": 8, "lambda_no_mu": 9, "beta_sqrt_ratio": 10, "mu_step_sign": 11,
+    "rho_step_sign": 12, "rho_psi_arg": 13, "newton_transpose": 14, "ab_order": 15,
+    "natgrad_sign": 16, "reparam_beta_nu": 17, "mu_shift_wt": 18, "lrate_growth": 19,
+    "lldec_no_halve": 20, "bflag_no_diag": 21, "alpha_over_t": 22, "phi_no_norm": 23,
+}
+
+
+def plant(name):
+    """Plants the named mistake in the oracle (None removes it).  Test-only."""
+    lib().orc_plant(0 if name is None else PLANTS[name])
 
 
 def pinv(A):
