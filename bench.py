@@ -11,15 +11,25 @@ reparameterisation) over all samples, which stay resident in HBM.
 Prints one JSON line.  Fields beyond the driver's contract:
   roofline      HBM roofline of the fused E-step, from HIP events recorded
                 around it on the engine stream during the timed steps; also
-                against the bandwidth of a measured device-to-device copy
+                against the bandwidth of this library's own streaming copy
+                kernel (amica_hbm_probe)
   gemm          algorithmic GEMM FLOPs (4 M n^2 T) over the two GEMMs' time,
                 against the dense MFMA peak of their operand type
-  cpu_baseline  the FP64 CPU oracle on a bounded sample of the workload
+  cpu_baseline  the FP64 CPU oracle (C++, OpenMP on all host cores) on the
+                whole recording, 2 iterations (median), with its FP32
+                variant beside it
+  accuracy_vs_fp64  after 50 iterations on 60k samples of the workload's
+                generator (its 40 blocks scaled to 1.5k samples), from the
+                same initial state: LL error against the FP64 oracle, per
+                model the Amari distance between the GPU's and the
+                oracle's W, the relative errors of alpha, mu, beta, rho,
+                and each run's Amari distance to the true mixing matrices
   ll_vs_fp32_path     BF16 / FP8 runs: the relative LL difference to the FP32
                 path (FP64 unmixing, held to the oracle by the parity tests)
                 after 50 iterations on the whole recording
-  ll_rel_err_vs_fp64  max over the first iterations on that sample (same
-                data, same initial state) of |LL_gpu - LL_fp64| / |LL_fp64|;
+  ll_rel_err_vs_fp64  max over the CPU baseline's iterations on the whole
+                recording (same data, same initial state) of
+                |LL_gpu - LL_fp64| / |LL_fp64|;
                 ll_rel_err_per_iteration lists them (the first is the
                 E-step alone, the second follows the first M-step)
 """
@@ -54,45 +64,125 @@ def estep_bytes(n, T, M, precision):
     return M * n * T * (4 + g_bytes)
 
 
-def cpu_baseline(w, x, st0, precision, iters=2, target_s=15.0):
-    """The FP64 oracle on the first T_s samples (T_s sized for ~target_s of
-    CPU work), and the GPU engine on the same samples for the LL check."""
+def _cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(w, x, st0, precision, iters=2):
+    """The reference's update math on the host: the FP64 oracle (and its
+    FP32 variant) on the whole recording, `iters` EM iterations each on all
+    host cores, timed per iteration (median); and the GPU engine on the same
+    data for the per-iteration LL check against FP64."""
     import cudamica_amd as ca
     from oracle import pyoracle as po
-    cores = os.cpu_count() or 1
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
-    n, M, m = w.n, w.M, w.m
-    # calibrate on a tiny slice, then size the sample
-    Tc = min(w.T, 20000)
-    xw, S, mean, ldetS = po.preprocess(x[:Tc], w.do_sphere)
-    cfg = po.default_config(n, Tc, M, m, maxiter=1, newt_start_iter=w.newt_start_iter)
-    t0 = time.perf_counter()
-    po.run(cfg, xw, ldetS, st0, nthreads=cores)
-    per_sample = (time.perf_counter() - t0) / Tc
-    Ts = int(min(w.T, max(Tc, target_s / iters / max(per_sample, 1e-12))))
-    Ts -= Ts % 4
-    log(f"cpu baseline: oracle FP64, {cores} threads, {iters} iterations on {Ts} samples")
-    xs = x[:Ts]
-    xw, S, mean, ldetS = po.preprocess(xs, w.do_sphere)
-    cfg = po.default_config(n, Ts, M, m, maxiter=iters, newt_start_iter=w.newt_start_iter)
-    t0 = time.perf_counter()
-    fin, logs = po.run(cfg, xw, ldetS, st0, nthreads=cores)
-    dt = time.perf_counter() - t0
-    its_full = 1.0 / ((dt / iters) * (w.T / Ts))
-    with ca.Engine(w.options(T=Ts, precision=precision, max_iter=iters)) as eng:
-        eng.load_raw(xs)
+    cores = _cores()
+    n, M, m, T = w.n, w.M, w.m, w.T
+    xw, S, mean, ldetS = po.preprocess(x, w.do_sphere)
+    res = {}
+    for bits in (64, 32):
+        log(f"cpu baseline: oracle FP{bits}, {cores} threads, {iters} iterations on all {T} samples")
+        st = {k: np.array(v, copy=True) for k, v in st0.items()}
+        per = []
+        logs = None
+        for k in range(iters):
+            cfg = po.default_config(n, T, M, m, maxiter=1, newt_start_iter=w.newt_start_iter)
+            t0 = time.perf_counter()
+            nst, ll, lr, _ = po.step(cfg, k + 1, xw, ldetS, st,
+                                     cfg.lrate0 if k == 0 else lr, ll_prev=0.0 if k == 0 else ll,
+                                     precision=bits, nthreads=cores)
+            per.append(time.perf_counter() - t0)
+            st = nst
+            if bits == 64:
+                logs = (logs or []) + [ll]
+        res[bits] = (float(np.median(per)), per, logs)
+    del xw
+    with ca.Engine(w.options(precision=precision, max_iter=iters)) as eng:
+        eng.load_raw(x)
         eng.preprocess(w.do_sphere)
         eng.set_state(st0)
         eng.iterate(iters)
         eng.sync()
         ll_gpu = eng.log(iters)["ll"]
-    rel = np.abs(ll_gpu - logs["ll"]) / np.abs(logs["ll"])
-    return {"value": its_full, "unit": "EM iters/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle, {iters} EM iterations on the first {Ts} of {w.T} samples, "
-                      f"{dt:.1f} s, scaled by T/{Ts}"}, rel, Ts
+    ll64 = np.array(res[64][2])
+    rel = np.abs(ll_gpu - ll64) / np.abs(ll64)
+    med64, per64, _ = res[64]
+    med32, per32, _ = res[32]
+    return {"value": 1.0 / med64, "unit": "EM iters/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (C++, OpenMP), {iters} EM iterations on all {T} samples, "
+                      f"median of {[round(t, 2) for t in per64]} s",
+            "fp32_variant": {"value": 1.0 / med32, "unit": "EM iters/s",
+                             "iteration_s": [round(t, 3) for t in per32]}}, rel
+
+
+def _amari(W, A):
+    """Amari index of P = W A, normalised to [0, 1]: 0 when P is a scaled
+    permutation (W unmixes A up to order and scale)."""
+    P = np.abs(W @ A)
+    n = P.shape[0]
+    r = (P / P.max(axis=1, keepdims=True)).sum(axis=1) - 1.0
+    c = (P / P.max(axis=0, keepdims=True)).sum(axis=0) - 1.0
+    return float((r.sum() + c.sum()) / (2.0 * n * (n - 1)))
+
+
+def accuracy_vs_fp64(w, x, st0, precision, T_acc=60000, iters=50):
+    """Error of the GPU path against the FP64 oracle after `iters` iterations
+    on T_acc samples of the workload's generator (its blocks scaled down),
+    both from the same data and initial state:
+    LL (last and max over the run), per model the Amari distance between the
+    two final W and the relative errors of alpha, mu, beta, rho; and each
+    model's Amari distance from the nearest true mixing matrix (of the ones
+    that occur in those samples), for both runs."""
+    import cudamica_amd as ca
+    from oracle import pyoracle as po
+    n, M, m = w.n, w.M, w.m
+    T_acc = min(T_acc, w.T)
+    # the workload's generator at T_acc samples with its block structure
+    # scaled down (the same number of blocks, each T_acc / T as long), so that
+    # every mixing matrix and so every model has data
+    spec = w.spec(T=T_acc)
+    if w.nmix > 1 and w.block > 0:
+        spec.block = max(1, T_acc // max(1, w.T // w.block))
+    xs, A_true, _ = ca.synth_data(spec, want_truth=True)
+    A_true = A_true.reshape(spec.nmix, n, n).transpose(0, 2, 1)
+    t0 = time.perf_counter()
+    xw, S, mean, ldetS = po.preprocess(xs, w.do_sphere)
+    cfg = po.default_config(n, T_acc, M, m, maxiter=iters, newt_start_iter=w.newt_start_iter)
+    fin, logs = po.run(cfg, xw, ldetS, st0, nthreads=_cores())
+    log(f"accuracy: FP64 oracle, {iters} iterations on {T_acc} samples in {time.perf_counter() - t0:.1f} s")
+    with ca.Engine(w.options(T=T_acc, precision=precision, max_iter=iters)) as eng:
+        eng.load_raw(xs)
+        eng.preprocess(w.do_sphere)
+        Sg = eng.sphere()[0].reshape(n, n).T
+        eng.set_state(st0)
+        eng.iterate(iters)
+        eng.sync()
+        glog = eng.log(iters)
+        gst = eng.get_state()
+    So = S.reshape(n, n).T
+    rel = np.abs(glog["ll"] - logs["ll"]) / np.abs(logs["ll"])
+    nn, nm = n * n, n * m
+    out = {"samples": T_acc, "iterations": iters, "ll_rel_last": float(rel[-1]),
+           "ll_rel_max": float(rel.max()), "ll_rel_max_iteration": int(np.argmax(rel)) + 1,
+           "decisions_equal": bool(np.array_equal(glog["lldec"], logs["lldec"])
+                                   and np.array_equal(glog["newton"], logs["newton"])),
+           "models": []}
+    for h in range(M):
+        Wg = gst["W"][h * nn:(h + 1) * nn].reshape(n, n).T
+        Wo = fin["W"][h * nn:(h + 1) * nn].reshape(n, n).T
+        sl = slice(h * nm, (h + 1) * nm)
+        d = {"amari_W_vs_oracle": float(_amari(Wg, np.linalg.inv(Wo))),
+             "amari_truth_gpu": float(min(_amari(Wg @ Sg, At) for At in A_true)),
+             "amari_truth_oracle": float(min(_amari(Wo @ So, At) for At in A_true))}
+        for k in ("alpha", "mu", "beta", "rho"):
+            d[f"{k}_rel"] = float(np.linalg.norm(gst[k][sl] - fin[k][sl]) / np.linalg.norm(fin[k][sl]))
+        out["models"].append(d)
+    worst = max(mm["amari_W_vs_oracle"] for mm in out["models"])
+    log(f"accuracy after {iters} iterations on {T_acc} samples: LL rel {rel[-1]:.2e} (max {rel.max():.2e}), "
+        f"worst W Amari vs oracle {worst:.2e}")
+    return out
 
 
 # dense MFMA peaks of the GEMMs' operand type (MI355X, TFLOP/s): split-BF16
@@ -101,33 +191,15 @@ GEMM_PEAK_TFLOPS = {16: 2500.0, 8: 2500.0, 32: 78.6}
 
 
 def measured_hbm_gbs(mib=2048, reps=5):
-    """Achievable HBM bandwidth: the best of `reps` device-to-device copies
-    of `mib` MiB (bytes read + bytes written per second), timed with HIP
-    events through the HIP runtime the engine uses."""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    nbytes = ctypes.c_size_t(mib << 20)
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-    ok = (hip.hipMalloc(ctypes.byref(a), nbytes) == 0 and hip.hipMalloc(ctypes.byref(b), nbytes) == 0
-          and hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
-          and hip.hipMemset(a, 1, nbytes) == 0)
-    best = 0.0
-    for _ in range(reps if ok else 0):
-        ms = ctypes.c_float(0.0)
-        if (hip.hipEventRecord(e0, None) != 0
-                or hip.hipMemcpyAsync(b, a, nbytes, 3, None) != 0     # device to device
-                or hip.hipEventRecord(e1, None) != 0 or hip.hipEventSynchronize(e1) != 0
-                or hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) != 0 or ms.value <= 0):
-            break
-        best = max(best, 2.0 * nbytes.value / (ms.value * 1e-3) / 1e9)
-    for p in (a, b):
-        if p.value:
-            hip.hipFree(p)
-    for e in (e0, e1):
-        if e.value:
-            hip.hipEventDestroy(e)
-    return best or None
+    """Achievable HBM bandwidth from this library's own streaming kernels
+    (amica_hbm_probe): a copy (bytes read + written per second) and a
+    read-only pass, best of `reps` over `mib` MiB buffers."""
+    import cudamica_amd as ca
+    try:
+        return ca.hbm_probe(0, mib, reps)
+    except Exception as ex:  # noqa: BLE001 -- reported, not fatal to the bench line
+        log(f"hbm probe failed: {ex}")
+        return None, None
 
 
 def ll_vs_fp32_path(w, x, st0, precision, iters=50):
@@ -256,6 +328,8 @@ def main():
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default=None)
     ap.add_argument("--T", type=int, default=None, help="override the sample count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the 50-iteration error report against the FP64 oracle")
     ap.add_argument("--no-ll-check", action="store_true",
                     help="skip the 50-iteration LL comparison with the FP32 path")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -309,7 +383,7 @@ def main():
 
     nbytes = estep_bytes(n, T, M, prec)
     achieved = nbytes / (kms[2] * 1e-3) / 1e9
-    hbm = measured_hbm_gbs()
+    hbm, hbm_read = measured_hbm_gbs()
     gemm_flop = 4.0 * M * n * n * T          # GEMM1 + GEMM2, algorithmic
     gemm_s = (kms[1] + kms[3]) * 1e-3
     gemm_peak = GEMM_PEAK_TFLOPS[prec]
@@ -332,7 +406,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(w.name, prec),
                      "kernel": "estep_kernel", "bytes_per_launch": nbytes,
                      "avg_ms": kms[2], "launches": cnt,
-                     "measured_copy_gbs": hbm,
+                     "measured_copy_gbs": hbm, "measured_read_gbs": hbm_read,
                      "frac_of_measured": achieved / hbm if hbm else None},
         "gemm": {"flop_per_iter": gemm_flop, "ms": gemm_s * 1e3,
                  "tflops": gemm_flop / gemm_s / 1e12,
@@ -343,12 +417,13 @@ def main():
     }
     if prec != 32 and not args.no_ll_check:
         out["ll_vs_fp32_path"] = ll_vs_fp32_path(w, x, st0, prec)
+    if not args.no_accuracy:
+        out["accuracy_vs_fp64"] = accuracy_vs_fp64(w, x, st0, prec)
     if not args.no_cpu_baseline:
-        cb, rel, Ts = cpu_baseline(w, x, st0, prec)
+        cb, rel = cpu_baseline(w, x, st0, prec)
         out["cpu_baseline"] = cb
         out["ll_rel_err_vs_fp64"] = float(np.max(rel))
         out["ll_rel_err_per_iteration"] = [float(r) for r in rel]
-        out["ll_check_samples"] = Ts
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -22,7 +22,7 @@ GXX = shutil.which("g++") or "g++"
 
 HIP_SRCS = [
     "kernels/estep.hip", "kernels/gemm.hip", "kernels/mstep.hip", "kernels/sphere.hip",
-    "kernels/engine.hip", "kernels/adjoint.hip", "host/device.hip",
+    "kernels/engine.hip", "kernels/adjoint.hip", "kernels/probe.hip", "host/device.hip",
 ]
 CXX_SRCS = [
     "host/params.cpp", "host/config.cpp", "host/helpers.cpp", "host/synth.cpp",

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -450,6 +451,15 @@ int amica_engine_center(amica_engine* e) {
     return 0;
 }
 
+/* ldetS enters the log-likelihood constant through the device state; data
+ * whitened after set_state must reach it there too */
+static int publish_ldetS(amica_engine* e) {
+    if (!e->have_state) return 0;
+    HIPCHK(hipMemcpy((char*)e->ds + offsetof(DevState, ldetS), &e->ldetS, sizeof(double),
+                     hipMemcpyHostToDevice));
+    return 0;
+}
+
 int amica_engine_whiten(amica_engine* e, int do_sphere) {
     if (!e) return fail(AMICA_E_INVALID, "null engine");
     if (!e->have_mean) return fail(AMICA_E_STATE, "whiten before center");
@@ -484,7 +494,7 @@ int amica_engine_whiten(amica_engine* e, int do_sphere) {
     if (se != hipSuccess) return fail(AMICA_E_DEVICE, std::string("sphering: ") + hipGetErrorString(se));
     HIPCHK(hipMemcpy(&e->ldetS, e->ldetS_d, sizeof(double), hipMemcpyDeviceToHost));
     e->have_white = 1;
-    return 0;
+    return publish_ldetS(e);
 }
 
 int amica_engine_preprocess(amica_engine* e, int do_sphere) {
@@ -530,7 +540,7 @@ int amica_engine_load_white(amica_engine* e, const double* xw, double ldetS) {
     HIPCHK(hipMemcpy(e->S, I.data(), I.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(e->mean, 0, e->o.n * sizeof(double)));
     e->have_white = 1;
-    return 0;
+    return publish_ldetS(e);
 }
 
 int amica_engine_get_white(amica_engine* e, float* x) {

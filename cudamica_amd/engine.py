@@ -75,6 +75,7 @@ _SIGNATURES = [
     ("amica_last_error", ctypes.c_char_p, []),
     ("amica_engine_debug_buffer", ctypes.c_int, [_V, ctypes.c_char_p, _V, ctypes.c_size_t]),
     ("amica_engine_cu_count", ctypes.c_int, [_V]),
+    ("amica_hbm_probe", ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _D, _D]),
     ("amica_synth_data", ctypes.c_int, [ctypes.POINTER(SynthSpec), _F, _D, _D, ctypes.c_int]),
     ("amica_synth_init", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
                                         ctypes.c_int, ctypes.c_double, _D, _D, _D, _D, _D, _D, _D]),
@@ -294,6 +295,14 @@ class Engine:
 
     def cu_count(self):
         return self.L.amica_engine_cu_count(self.h)
+
+
+def hbm_probe(device=0, mib=2048, reps=5):
+    """Achievable HBM bandwidth (GB/s) of this library's streaming copy
+    (bytes read + written) and read-only kernels over `mib` MiB buffers."""
+    c, r = np.zeros(1), np.zeros(1)
+    _check(load_library().amica_hbm_probe(device, mib << 20, reps, _d(c), _d(r)), "amica_hbm_probe")
+    return float(c[0]), float(r[0])
 
 
 def synth_spec(n, T, seed, nmix=1, block=0, alternate=0, rho=(1.0, 2.0), sub_frac=0.0,

@@ -136,6 +136,14 @@ const char* amica_last_error(void);
 /* Number of compute units of the engine's device. */
 int  amica_engine_cu_count(amica_engine* eng);
 
+/* Achievable HBM bandwidth on `device`, measured with this library's own
+ * streaming kernels over two buffers of `bytes` each (best of `reps`
+ * timed passes after one warm-up): copy_gbs counts bytes read + written,
+ * read_gbs bytes read by a read-only pass.  No counterpart in the
+ * reference (it prints only per-iteration times, amica.cu:2538-2540);
+ * the yardstick beside the nominal HBM peak for the kernel figures. */
+int  amica_hbm_probe(int device, size_t bytes, int reps, double* copy_gbs, double* read_gbs);
+
 #ifdef __cplusplus
 }
 #endif

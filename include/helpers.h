@@ -32,8 +32,10 @@ extern "C" {
  * no sample blocks of its own; block_size and blocks_per_gpu_per_model are
  * still computed for the reference's `blocks` table. */
 error recalc_config(void);
-/* Measures the E-step launch shape for the current configuration on the
- * given device (the reference's heuristic table, helpers.cu:55-511). */
+/* The reference fills its heuristic launch-shape table here
+ * (helpers.cu:55-511).  This engine sizes every grid at creation from the
+ * occupancy of its kernels on the selected device, so there is nothing to
+ * fill: kept for the reference's call sequence, returns SUCCESS. */
 error autotune(natural device);
 #ifdef __cplusplus
 error loadMatrix(real* data, natural rows, natural elems, char* filename, bool isdouble = false);

@@ -25,3 +25,13 @@ def test_more_than_one_gpu_is_refused():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_amari_index_is_zero_exactly_for_scaled_permutations():
+    import numpy as np
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(6, 6))
+    P = np.eye(6)[rng.permutation(6)] * rng.uniform(0.5, 2.0, size=6) * rng.choice([-1, 1], size=6)
+    W = P @ np.linalg.inv(A)
+    assert bench._amari(W, A) < 1e-12
+    assert bench._amari(np.linalg.inv(A + 0.1 * rng.normal(size=(6, 6))), A) > 1e-3

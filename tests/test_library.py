@@ -100,3 +100,25 @@ def test_ll_entry_points_validate_before_touching_gpu(built):
     assert L.amica_ll_forward(64, 1, 10, 4, 1, 3, *p, 0.0, None, None) == -3     # no ll output
     assert b"ll output" in L.amica_last_error()
     assert L.amica_ll_adjoint(64, 1, 10, 4, 1, 3, *p, 0.0, None, None, *p, None) == -3  # no dll
+
+
+REF_MAIN = "/root/reference/src/cudaamica.cu"
+
+
+def test_reference_main_builds_against_these_headers(built, tmp_path):
+    """The reference's own main (src/cudaamica.cu:17-43), unchanged, compiles
+    as C99 against include/ and links against libcudamica_amd.so: the
+    boundary keeps its names and signatures.  Skipped where the reference
+    tree is absent."""
+    import pytest
+    if not os.path.exists(REF_MAIN):
+        pytest.skip("reference sources not present")
+    exe = str(tmp_path / "cudaamica_ref_main")
+    r = subprocess.run(["gcc", "-x", "c", "-std=c99", "-Wall", "-I", os.path.join(ROOT, "include"),
+                        REF_MAIN, "-x", "none", "-o", exe, "-L", os.path.dirname(ca.LIB_PATH),
+                        "-lcudamica_amd", f"-Wl,-rpath,{os.path.dirname(ca.LIB_PATH)}",
+                        "-Wl,-rpath,/opt/rocm/lib"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert "configuration file is mandatory" in r.stdout
