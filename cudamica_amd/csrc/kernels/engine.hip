@@ -89,6 +89,10 @@ struct amica_engine {
     /* partials */
     int nblocks = 0;
     int nblocks_cap = 0;           /* E-step partial slots allocated */
+    /* split E-step (all models' channel blocks exceed one workgroup) */
+    int split = 0, sp_mg = 0, sp_cwg = 0, sp_ncg = 0, sp_groups = 1;
+    float* lq = nullptr;           /* [T32][M][ncg] per-group channel sums of q */
+    float* vq = nullptr;           /* [T32][M] model posteriors */
     int nchunks_cap = 0;           /* statistic-GEMM partial slots allocated */
     int cu_budget = 0;             /* CUs the grids are sized for (default: all) */
     int estep_wg_per_cu = 0;
@@ -140,18 +144,30 @@ struct amica_engine {
     void size_grids(int budget) {
         cu_budget = std::max(1, std::min(budget, cus));
         const long T = o.T;
-        const long target = (long)cu_budget * estep_wg_per_cu;
+        long target = (long)cu_budget * estep_wg_per_cu;
+        if (split) {
+            /* grid.y runs the workgroup groups of one sample chunk; the
+             * partial slots (one per chunk) stay within 2 GiB */
+            target = std::max(1L, target / sp_groups);
+            const long by_mem = std::max(1L, (2L << 30) / (long)(pstride_for() * sizeof(double)));
+            target = std::min(target, by_mem);
+        }
         nblocks = (int)std::max(1L, std::min(target, (T + 31) / 32));
         chunk = (T + nblocks - 1) / nblocks;
         chunk = ((chunk + 31) / 32) * 32;     /* whole packed 8-sample groups */
         nblocks = (int)((T + chunk - 1) / chunk);
-        nchunks = bf16 ? gx_bf16_chunks(T, NP, cu_budget) : gx_f32_chunks(T, NP, ldb, cu_budget);
+        nchunks = bf16 ? gx_bf16_chunks(T, NP, ldb, cu_budget) : gx_f32_chunks(T, NP, ldb, cu_budget);
         if (gexec) {   /* the captured grids are stale */
             (void)hipGraphExecDestroy(gexec);
             (void)hipGraphDestroy(graph);
             gexec = nullptr;
             graph = nullptr;
         }
+    }
+
+    size_t pstride_for() const {
+        const int NS = kStatsPerJ * o.m + kStatsPerI;
+        return (((size_t)o.M * NS * NP + o.M + 1) + 31) & ~(size_t)31;
     }
 
     MstepArgs margs() const {
@@ -186,6 +202,8 @@ struct amica_engine {
         a.x = xw; a.g = g; a.cst = cst; a.lconst = lconst; a.part = part; a.ltall = lt;
         a.T = o.T; a.chunk = chunk; a.pstride = pstride;
         a.n = o.n; a.NP = NP; a.M = o.M; a.CW = CW; a.ldb = ldb; a.g_bf16 = bf16;
+        a.split = split; a.mg = sp_mg; a.cwg = sp_cwg; a.ncg = sp_ncg; a.mj = o.m;
+        a.lq = lq; a.vq = vq;
         return a;
     }
 
@@ -260,12 +278,8 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     if (o.m < 1 || o.m > AMICA_MAX_MJ) return fail(AMICA_E_INVALID, "m must be in [1, 8]");
     if (o.precision != AMICA_PREC_FP32 && o.precision != AMICA_PREC_BF16 && o.precision != AMICA_PREC_FP8)
         return fail(AMICA_E_INVALID, "precision must be 32 (fp32), 16 (bf16) or 8 (fp8)");
-    if (o.precision == AMICA_PREC_FP8 && ((o.n + 63) / 64) * 64 > 256)
-        return fail(AMICA_E_INVALID, "fp8 storage needs n <= 256");
     const int NP = ((o.n + 63) / 64) * 64;
     const int CW = NP / 64;
-    if (o.M * CW > kMaxEstepWaves)
-        return fail(AMICA_E_INVALID, "M * ceil(n/64) must be <= 8 for the fused E-step");
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (o.device < 0 || o.device >= ndev) return fail(AMICA_E_INVALID, "no such HIP device");
@@ -277,6 +291,10 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     e->NP = NP;
     e->CW = CW;
     e->ldb = o.M * NP;
+    /* the fused E-step holds every model's channel blocks in one workgroup
+     * of at most kMaxEstepWaves waves; larger shapes run the split launch */
+    e->split = o.M * CW > kMaxEstepWaves;
+    if (e->split) estep_split_shape(o.M, CW, &e->sp_mg, &e->sp_cwg, &e->sp_ncg, &e->sp_groups);
     e->bf16 = (o.precision == AMICA_PREC_BF16 || o.precision == AMICA_PREC_FP8);
     e->fp8 = (o.precision == AMICA_PREC_FP8);
     hipDeviceProp_t prop;
@@ -317,9 +335,13 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     } else {
         ALLOC(e->g, (size_t)T * e->ldb * sizeof(float));
     }
+    if (e->split) {
+        ALLOC(e->lq, (size_t)T32 * M * e->sp_ncg * sizeof(float));
+        ALLOC(e->vq, (size_t)T32 * M * sizeof(float));
+    }
     /* E-step grid: exactly as many workgroups as the CUs hold at once, each
      * streaming one contiguous range of samples. */
-    const int waves = M * CW;
+    const int waves = e->split ? e->sp_mg * e->sp_cwg : M * CW;
     {
         const int rc = estep_configure(m, estep_lds_bytes(m, 64 * waves));
         if (rc) {
@@ -331,19 +353,17 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
         amica_engine_destroy(e);
         return fail(AMICA_E_DEVICE, "M-step LDS configuration failed");
     }
-    const int wg_per_cu = estep_blocks_per_cu(m, !e->bf16, 64 * waves, M == 1);
+    const int wg_per_cu = estep_blocks_per_cu(m, !e->bf16, 64 * waves, M == 1 && !e->split, e->split != 0);
     if (wg_per_cu < 1) {
         amica_engine_destroy(e);
         return fail(AMICA_E_DEVICE, "E-step occupancy query failed");
     }
     e->estep_wg_per_cu = wg_per_cu;
     /* partial slots for the largest grids (the whole device) */
+    e->pstride = e->pstride_for();
     e->size_grids(e->cus);
     e->nblocks_cap = e->nblocks;
     e->nchunks_cap = e->nchunks;
-    const int NS = kStatsPerJ * m + kStatsPerI;
-    e->pstride = (size_t)M * NS * NP + M + 1;
-    e->pstride = (e->pstride + 31) & ~(size_t)31;
     ALLOC(e->part, (size_t)e->nblocks_cap * e->pstride * sizeof(double));
     ALLOC(e->gxpart, (size_t)e->nchunks_cap * e->ldb * NP * sizeof(double));
     ALLOC(e->raw, e->pstride * sizeof(double));
@@ -410,7 +430,7 @@ int amica_engine_destroy(amica_engine* e) {
                     e->cst64, e->stats, e->sigma2, e->cnew, e->phis,
                     e->kappa, e->lambda, e->Bm, e->work, e->lu, e->bflag, e->newton, e->ds,
                     e->log_ll, e->log_lrate, e->log_vsum, e->log_bflag, e->log_newton,
-                    e->log_lldec, e->ltall, e->S, e->mean, e->ldetS_d};
+                    e->log_lldec, e->ltall, e->S, e->mean, e->ldetS_d, e->lq, e->vq};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& s : e->evs)

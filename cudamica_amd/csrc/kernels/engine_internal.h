@@ -47,6 +47,9 @@ constexpr int kEstepRowPad = 16;
  * sqrt(beta) and -sqrt(beta) mu, so that y is formed in FP64. */
 enum { kCst64Sb = 0, kCst64Nsbmu = 1, kNumCst64 = 2 };
 
+/* E-step launches (estep.hip): fused, or the split passes */
+enum { kEstepFused = 0, kEstepLsum = 1, kEstepStats = 2 };
+
 struct EstepArgs {
     const float* b;        /* [T][ldb] (FP32 unmixing) */
     const double* b64;     /* [T][ldb] (FP64 unmixing), or null */
@@ -62,16 +65,27 @@ struct EstepArgs {
     size_t pstride;
     int n, NP, M, CW, ldb;
     int g_bf16;
+    /* split launch (M CW > kMaxEstepWaves): workgroups of mg models x cwg
+     * channel blocks, blockIdx.y = model group * ncg + channel group; lq
+     * [T][M][ncg] holds each group's sum of q (log2 units), vq [T][M] the
+     * model posteriors */
+    int split, mg, cwg, ncg, mj;
+    float* lq;
+    float* vq;
 };
 
+/* Workgroup shape of the split E-step for M models of CW channel blocks:
+ * models per group, channel blocks per group, channel groups, groups. */
+void estep_split_shape(int M, int CW, int* mg, int* cwg, int* ncg, int* ngroups);
 int launch_estep(const EstepArgs& a, int m, int nblocks, hipStream_t st);
 /* Allows the E-step variants for m components `lds_bytes` of dynamic LDS. */
 int estep_configure(int m, size_t lds_bytes);
 /* Dynamic LDS of one E-step workgroup of `threads` threads. */
 size_t estep_lds_bytes(int m, int threads);
 /* Resident E-step workgroups per CU (registers and LDS), or -1; m1 for the
- * single-model kernel (M = 1, no per-sample L recorded). */
-int estep_blocks_per_cu(int m, bool b64, int threads, bool m1);
+ * single-model kernel (M = 1, no per-sample L recorded), split for the
+ * split launch's statistics pass. */
+int estep_blocks_per_cu(int m, bool b64, int threads, bool m1, bool split = false);
 
 /* GEMMs ------------------------------------------------------------------ */
 /* b[t][c] = sum_k x[t][k] W[c][k] - wc[c]    (c < ldb, k < NP) */
@@ -96,7 +110,7 @@ int launch_gx_bf16(const uint16_t* g8, const uint16_t* x8, double* part, long T,
                    int nchunks, hipStream_t st);
 /* Number of sample chunks (partials) launch_gx_bf16 / launch_gx_f32 should
  * be given for a budget of `cus` CUs (nondecreasing in cus). */
-int gx_bf16_chunks(long T, int NP, int cus);
+int gx_bf16_chunks(long T, int NP, int ldb, int cus);
 int gx_f32_chunks(long T, int NP, int ldb, int cus);
 /* x8[(t/8) NP + i][t%8] = xhi[t][i] (the packed B operand of the BF16
  * statistic GEMM); padding stays as allocated (zero). */

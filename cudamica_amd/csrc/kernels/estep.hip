@@ -52,6 +52,8 @@
  * weight |y|^(rho-2) is singular for rho < 2 -- keeps full relative
  * precision.
  */
+#include <algorithm>
+
 #include "common.h"
 #include "engine_internal.h"
 
@@ -112,7 +114,16 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2){a, b}, bf2));
 }
 
-template <int MJ, bool B64>
+/* MODE selects the launch the kernel serves:
+ *   kEstepFused  all M models x CW channel blocks in one workgroup (M CW <= 8):
+ *                the cross-channel and cross-model sums stay on chip;
+ *   kEstepLsum   split launch, first pass: a workgroup holds a group of models
+ *                x channel blocks (blockIdx.y), and writes each model's sum of
+ *                q over its channel blocks per sample (lq);
+ *   kEstepStats  split launch, last pass: the densities again, the model
+ *                posteriors read from vq (written between the passes by
+ *                estep_post_kernel), then the statistics and g. */
+template <int MJ, bool B64, int MODE>
 __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     /* samples per barrier tile: 2, or 1 where m >= 4 so that the per-sample
      * values stay within 128 VGPRs (two workgroups per CU) */
@@ -120,12 +131,28 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
     constexpr int kFlush = 32 / kS;     /* tiles summed in FP32 registers (32 samples) */
     constexpr int NSJ = kStatsPerJ;
     constexpr int NS = NSJ * MJ + kStatsPerI;
+    constexpr bool kSplit = MODE != kEstepFused;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   /* wave-uniform */
-    const int h = w / a.CW;
-    const int cb = w - h * a.CW;
+    int h, cb, hg = 0, cg = 0;
+    bool wreal = true;                  /* wave-uniform: the wave owns real columns */
+    if constexpr (kSplit) {
+        hg = blockIdx.y / a.ncg;
+        cg = blockIdx.y - hg * a.ncg;
+        const int hl = w / a.cwg;
+        h = hg * a.mg + hl;
+        cb = cg * a.cwg + (w - hl * a.cwg);
+        wreal = h < a.M && cb < a.CW;
+        /* a wave past the last model or channel block reads a real column and
+         * runs as padding channels (q = 0, no statistics, nothing written) */
+        h = min(h, a.M - 1);
+        cb = min(cb, a.CW - 1);
+    } else {
+        h = w / a.CW;
+        cb = w - h * a.CW;
+    }
     const int i = cb * 64 + lane;
-    const bool act = i < a.n;
+    const bool act = wreal && i < a.n;
 
     /* per-row channel sums of one tile, double buffered:
      * [buf][wave][sample][row of 16 lanes] (a sample's 4 rows in 16 bytes),
@@ -181,8 +208,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             }
         }
     }
+    if constexpr (MODE != kEstepLsum) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) tot2[s] = 0.f;
+        for (int s = 0; s < NS; ++s) tot2[s] = 0.f;
+    }
 
     float acc[NS];
 #pragma unroll
@@ -212,15 +241,24 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
      * offset is fixed, the row's a scalar (launch_estep bounds it below
      * 2^31), so a load costs no address arithmetic on the vector ALU */
     constexpr int kEb = B64 ? 8 : 4;
-    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)((const char*)(B64 ? (const void*)a.b64 : (const void*)a.b) + (size_t)t0 * a.ldb * kEb),
-        (short)0, (int)0xffffffff, kBufferRsrcWord3);
-    const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(a.x + (size_t)t0 * a.NP), (short)0, (int)0xffffffff, kBufferRsrcWord3);
+    const char* const bbase = (const char*)(B64 ? (const void*)a.b64 : (const void*)a.b);
+    auto rsrc_b = [&](long t) {
+        return __builtin_amdgcn_make_buffer_rsrc((void*)(bbase + (size_t)t * a.ldb * kEb), (short)0,
+                                                 (int)0xffffffff, kBufferRsrcWord3);
+    };
+    auto rsrc_x = [&](long t) {
+        return __builtin_amdgcn_make_buffer_rsrc((void*)(a.x + (size_t)t * a.NP), (short)0,
+                                                 (int)0xffffffff, kBufferRsrcWord3);
+    };
+    /* rows are addressed relative to tbase: the workgroup's first row (fused
+     * launch, whose row offsets launch_estep bounds), or in the split
+     * launches -- whose rows may be far wider -- the current 8-sample group */
+    long tbase = t0;
+    __amdgpu_buffer_rsrc_t rb = rsrc_b(t0), rx = rsrc_x(t0);
     const int vob = (int)bcol * kEb, vox = i * 4;
     const int browb = a.ldb * kEb, xrowb = a.NP * 4;
     auto load_tile = [&](long t, int s) {
-        const int r = (int)(t - t0);
+        const int r = (int)(t - tbase);
         if constexpr (B64)
             bn64[s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rb, vob, r * browb, 0));
         else
@@ -232,6 +270,12 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 
     constexpr int kTPG = 8 / kS;               /* tiles per packed 8-sample group */
     for (long tg = t0; tg < t1; tg += 8) {
+        if constexpr (kSplit) {
+            /* loads already in flight keep the resources they were issued with */
+            tbase = tg;
+            rb = rsrc_b(tg);
+            rx = rsrc_x(tg);
+        }
         /* the group's packed g words (2 samples each); tiles past t1 in the
          * last group run with posterior 0 and so give g = 0 */
         uint32_t gw[4] = {0u, 0u, 0u, 0u};
@@ -298,6 +342,13 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 }
                 RS[s] = fast_rcp(se);
             }
+            const int nval = (int)min(t1 - tb, (long)kS);    /* samples of the tile before t1 */
+            float GV[kS], VV[kS];
+            if constexpr (MODE == kEstepStats) {
+                /* the posteriors of the split launch's middle pass */
+#pragma unroll
+                for (int s = 0; s < kS; ++s) VV[s] = s < nval ? a.vq[(size_t)(tb + s) * a.M + h] : 0.f;
+            } else {
 #pragma unroll
             for (int s = 0; s < kS; ++s) QS[s] = row_sum_lane15(QS[s]);
             if ((lane & 15) == 15) {
@@ -305,10 +356,26 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 for (int s = 0; s < kS; ++s) Lp[buf][w][s][lane >> 4] = QS[s];
             }
             __syncthreads();
+            if constexpr (MODE == kEstepLsum) {
+                /* wave 0: lane 16 s + hl adds the row sums of local model hl
+                 * for sample s and writes them for the posterior pass */
+                if (w == 0) {
+                    const int hl = lane & 15, s = lane >> 4;
+                    const int hh = hg * a.mg + hl;
+                    if (s < kS && s < nval && hl < a.mg && hh < a.M) {
+                        float l = 0.f;
+                        for (int c = 0; c < a.cwg; ++c)
+#pragma unroll
+                            for (int row = 0; row < 4; ++row) l += Lp[buf][hl * a.cwg + c][s][row];
+                        a.lq[((size_t)(tb + s) * a.M + hh) * a.ncg + cg] = l;
+                    }
+                }
+                buf ^= 1;
+                continue;
+            } else {
             /* ---- model posteriors, by wave 0 alone: lane hh*kS + s holds model
              * hh, sample s; the other waves read them from LDS after a second
              * barrier ---- */
-            const int nval = (int)min(t1 - tb, (long)kS);    /* samples of the tile before t1 */
             if (w == 0) {
                 /* the other waves wait for this one at the next barrier: let it
                  * issue ahead of the other workgroup's waves on its SIMD */
@@ -351,8 +418,6 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
-            /* ---- phase 2: responsibilities and statistics ---- */
-            float GV[kS], VV[kS];
             if constexpr (kS == 2) {                      /* both samples in one 8-byte read */
                 const float2 vp = *(const float2*)&Vp[buf][h * 2];
                 VV[0] = vp.x;
@@ -360,6 +425,9 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             } else {
                 VV[0] = Vp[buf][h];
             }
+            }
+            }
+            /* ---- phase 2: responsibilities and statistics ---- */
 #pragma unroll
             for (int s = 0; s < kS; ++s) {
                 const long t = tb + s;
@@ -387,7 +455,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 acc[NSJ * MJ + 1] = fmaf(v, XV[s], acc[NSJ * MJ + 1]);
                 acc[NSJ * MJ + 2] += g;
                 GV[s] = g;
-                if (!a.g_bf16 && s < nval) ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
+                if (!a.g_bf16 && s < nval && wreal) ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
             }
             if constexpr (kS == 2) gw[sub] = pack_bf16x2(GV[0], GV[1]);
             else if (sub & 1) gw[sub >> 1] = pack_bf16x2(gpend, GV[0]);
@@ -405,21 +473,24 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         const double v = (double)tot2[s];
-                        l3[s] = l3_started ? l3[s] + v : v;
+                        if (wreal) l3[s] = l3_started ? l3[s] + v : v;
                         tot2[s] = 0.f;
                     }
                     l3_started = true;
                 }
             }
         }
-        if (a.g_bf16) g8[(size_t)(tg >> 3) * a.ldb + bcol] = make_uint4(gw[0], gw[1], gw[2], gw[3]);
+        if constexpr (MODE != kEstepLsum)
+            if (a.g_bf16 && wreal) g8[(size_t)(tg >> 3) * a.ldb + bcol] = make_uint4(gw[0], gw[1], gw[2], gw[3]);
     }
+    if constexpr (MODE == kEstepLsum) return;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const double v = (double)tot2[s] + (double)acc[s];
         const double r = l3_started ? l3[s] + v : v;
-        l3[s] = act ? r : 0.0;
+        if (wreal) l3[s] = act ? r : 0.0;
     }
+    if constexpr (MODE == kEstepStats) return;   /* vsum and the LL sum: estep_post_kernel */
     /* per-model vsum (lanes hh*kS + s of wave 0) and the LL sum (lanes s) */
     if (w == 0) {
         const size_t tail = (size_t)a.M * NS * a.NP;
@@ -638,16 +709,104 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
     }
 }
 
+/* The split launch's middle pass: per sample, the model log-likelihoods
+ * L_h = (log gm_h + ldet_h + ldetS) + sum of q over all channels (the
+ * channel-group sums of the first pass), their log-sum-exp (the sample's
+ * log-likelihood, as updatell, kernels.cu:376-450) and the posteriors
+ * v_h = exp(L_h - logsumexp) for the statistics pass.  One workgroup per
+ * sample chunk (the chunks of the other passes): vsum per model and the LL
+ * sum go to the tail of the chunk's partial slot, in FP64. */
+__global__ __launch_bounds__(256) void estep_post_kernel(EstepArgs a) {
+    __shared__ float dc2[AMICA_MAX_M];
+    __shared__ double red[4][AMICA_MAX_M + 1];
+    const int M = a.M, ncg = a.ncg;
+    if (threadIdx.x < M) {
+        double c0 = a.lconst[0];
+        for (int hh = 1; hh < M; ++hh) c0 = fmax(c0, a.lconst[hh]);
+        dc2[threadIdx.x] = (float)((a.lconst[threadIdx.x] - c0) * 1.4426950408889634);
+    }
+    __syncthreads();
+    const long t0 = (long)blockIdx.x * a.chunk;
+    const long t1 = min(a.T, t0 + a.chunk);
+    double ll = 0.0, vs[AMICA_MAX_M];
+#pragma unroll
+    for (int hh = 0; hh < AMICA_MAX_M; ++hh) vs[hh] = 0.0;
+    for (long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+        float l[AMICA_MAX_M];
+        float lmax = -3.0e38f;
+#pragma unroll
+        for (int hh = 0; hh < AMICA_MAX_M; ++hh) {
+            if (hh < M) {
+                float v = dc2[hh];
+                const float* q = a.lq + ((size_t)t * M + hh) * ncg;
+                for (int c = 0; c < ncg; ++c) v += q[c];
+                l[hh] = v;
+                lmax = fmaxf(lmax, v);
+            }
+        }
+        float esum = 0.f;
+#pragma unroll
+        for (int hh = 0; hh < AMICA_MAX_M; ++hh)
+            if (hh < M) esum += fast_exp2(l[hh] - lmax);
+        const float r = fast_rcp(esum);
+#pragma unroll
+        for (int hh = 0; hh < AMICA_MAX_M; ++hh) {
+            if (hh < M) {
+                const float v = fast_exp2(l[hh] - lmax) * r;
+                a.vq[(size_t)t * M + hh] = v;
+                vs[hh] += (double)v;
+                if (a.ltall) a.ltall[(size_t)hh * a.T + t] = l[hh] * (float)AMICA_LN2;
+            }
+        }
+        ll += (double)(lmax + fast_log2(esum)) * AMICA_LN2;
+    }
+    /* fixed-order block sums: waves, then the 4 wave totals */
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double lls = wave_sum_f64(ll);
+    if (lane == 0) red[w][M] = lls;
+#pragma unroll
+    for (int hh = 0; hh < AMICA_MAX_M; ++hh) {
+        if (hh < M) {
+            const double v = wave_sum_f64(vs[hh]);
+            if (lane == 0) red[w][hh] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x <= M) {
+        const double tot = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+        const int NS = kStatsPerJ * a.mj + kStatsPerI;
+        a.part[(size_t)blockIdx.x * a.pstride + (size_t)M * NS * a.NP + threadIdx.x] = tot;
+    }
+}
+
 }  // namespace
+
+void estep_split_shape(int M, int CW, int* mg, int* cwg, int* ncg, int* ngroups) {
+    /* channel blocks per group: all of a model's if they fit one workgroup,
+     * else the fewest groups of at most kMaxEstepWaves blocks, balanced */
+    const int g = (CW + kMaxEstepWaves - 1) / kMaxEstepWaves;
+    const int c = (CW + g - 1) / g;
+    const int models = std::max(1, std::min(M, kMaxEstepWaves / c));
+    *cwg = c;
+    *ncg = g;
+    *mg = models;
+    *ngroups = ((M + models - 1) / models) * g;
+}
 
 int estep_configure(int MJ, size_t lds_bytes) {
     hipError_t r = hipSuccess;
 #define AMICA_ESTEP_ATTR(J)                                                                       \
     case J:                                                                                       \
-        r = hipFuncSetAttribute((const void*)estep_kernel<J, true>,                              \
+        r = hipFuncSetAttribute((const void*)estep_kernel<J, true, kEstepFused>,                 \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);      \
         if (r == hipSuccess)                                                                      \
-            r = hipFuncSetAttribute((const void*)estep_kernel<J, false>,                         \
+            r = hipFuncSetAttribute((const void*)estep_kernel<J, false, kEstepFused>,            \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
+        if (r == hipSuccess)                                                                      \
+            r = hipFuncSetAttribute((const void*)estep_kernel<J, true, kEstepStats>,             \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
+        if (r == hipSuccess)                                                                      \
+            r = hipFuncSetAttribute((const void*)estep_kernel<J, false, kEstepStats>,            \
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
         if (r == hipSuccess)                                                                      \
             r = hipFuncSetAttribute((const void*)estep_m1_kernel<J, true>,                       \
@@ -675,7 +834,7 @@ size_t estep_lds_bytes(int MJ, int threads) {
     return (size_t)((kStatsPerJ * MJ + kStatsPerI) | 1) * threads * sizeof(float);
 }
 
-int estep_blocks_per_cu(int MJ, bool b64, int threads, bool m1) {
+int estep_blocks_per_cu(int MJ, bool b64, int threads, bool m1, bool split) {
     int nb = 0;
     const size_t shm = estep_lds_bytes(MJ, threads);
     hipError_t r = hipErrorInvalidValue;
@@ -686,11 +845,16 @@ int estep_blocks_per_cu(int MJ, bool b64, int threads, bool m1) {
                                                                    threads, shm)                  \
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_m1_kernel<J, false>,\
                                                                    threads, shm);                 \
+        else if (split)                                                                           \
+            r = b64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(                               \
+                          &nb, estep_kernel<J, true, kEstepStats>, threads, shm)                  \
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(                               \
+                          &nb, estep_kernel<J, false, kEstepStats>, threads, shm);                \
         else                                                                                      \
-            r = b64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_kernel<J, true>,    \
-                                                                   threads, shm)                  \
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, estep_kernel<J, false>,   \
-                                                                   threads, shm);                 \
+            r = b64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(                               \
+                          &nb, estep_kernel<J, true, kEstepFused>, threads, shm)                  \
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(                               \
+                          &nb, estep_kernel<J, false, kEstepFused>, threads, shm);                \
         break;
     switch (MJ) {
         AMICA_ESTEP_OCC(1)
@@ -707,7 +871,39 @@ int estep_blocks_per_cu(int MJ, bool b64, int threads, bool m1) {
     return r == hipSuccess ? nb : -1;
 }
 
+static int launch_estep_split(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
+    int mg = 0, cwg = 0, ncg = 0, ngroups = 0;
+    estep_split_shape(a.M, a.CW, &mg, &cwg, &ncg, &ngroups);
+    if (a.mg != mg || a.cwg != cwg || a.ncg != ncg || !a.lq || !a.vq || a.mj != MJ || (a.chunk % 8) != 0)
+        return -1;
+    const dim3 grid(nblocks, ngroups), block(64 * mg * cwg);
+    const size_t shm = estep_lds_bytes(MJ, (int)block.x);
+    const bool b64 = a.b64 != nullptr;
+#define AMICA_ESTEP_SPLIT(J)                                                                          \
+    case J:                                                                                           \
+        if (b64) hipLaunchKernelGGL((estep_kernel<J, true, kEstepLsum>), grid, block, 0, st, a);      \
+        else hipLaunchKernelGGL((estep_kernel<J, false, kEstepLsum>), grid, block, 0, st, a);         \
+        hipLaunchKernelGGL(estep_post_kernel, dim3(nblocks), dim3(256), 0, st, a);                   \
+        if (b64) hipLaunchKernelGGL((estep_kernel<J, true, kEstepStats>), grid, block, shm, st, a);   \
+        else hipLaunchKernelGGL((estep_kernel<J, false, kEstepStats>), grid, block, shm, st, a);      \
+        break;
+    switch (MJ) {
+        AMICA_ESTEP_SPLIT(1)
+        AMICA_ESTEP_SPLIT(2)
+        AMICA_ESTEP_SPLIT(3)
+        AMICA_ESTEP_SPLIT(4)
+        AMICA_ESTEP_SPLIT(5)
+        AMICA_ESTEP_SPLIT(6)
+        AMICA_ESTEP_SPLIT(7)
+        AMICA_ESTEP_SPLIT(8)
+        default: return -1;
+    }
+#undef AMICA_ESTEP_SPLIT
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
+    if (a.split) return launch_estep_split(a, MJ, nblocks, st);
     if (a.M * a.CW > kMaxEstepWaves || a.M > 8 || (a.chunk % 8) != 0) return -1;
     /* byte offsets of the rows a workgroup reads (its chunk and the next
      * tile past it) must fit the 31-bit scalar offsets of its buffer loads */
@@ -722,8 +918,8 @@ int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
             if (b64) hipLaunchKernelGGL((estep_m1_kernel<J, true>), grid, block, shm, st, a); \
             else hipLaunchKernelGGL((estep_m1_kernel<J, false>), grid, block, shm, st, a);    \
         } else {                                                                              \
-            if (b64) hipLaunchKernelGGL((estep_kernel<J, true>), grid, block, shm, st, a);    \
-            else hipLaunchKernelGGL((estep_kernel<J, false>), grid, block, shm, st, a);       \
+            if (b64) hipLaunchKernelGGL((estep_kernel<J, true, kEstepFused>), grid, block, shm, st, a); \
+            else hipLaunchKernelGGL((estep_kernel<J, false, kEstepFused>), grid, block, shm, st, a);    \
         }                                                                                     \
         break;
     switch (MJ) {

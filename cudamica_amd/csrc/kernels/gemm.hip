@@ -180,6 +180,51 @@ __device__ __forceinline__ i16x8 fp8x8_to_bf16x8(uint2 q) {
     return r;
 }
 
+/* FP8 unmixing for any NP (the register-resident kernel below covers
+ * NP <= 256): b = q W_hi + q W_lo - W~ c, q the e4m3 data converted exactly
+ * to bf16 in registers; W fragments straight from global memory (L2-
+ * resident).  Wave tile 16 samples x 64 columns, workgroup 64 x 64. */
+__global__ __launch_bounds__(256) void unmix_fp8_tiled_kernel(const uint8_t* __restrict__ xq,
+                                                              const uint16_t* __restrict__ Whi,
+                                                              const uint16_t* __restrict__ Wlo,
+                                                              const float* __restrict__ wc,
+                                                              float* __restrict__ b, long T, int NP,
+                                                              int ldb) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long t0 = (long)blockIdx.x * 64 + w * 16;
+    const int c0 = blockIdx.y * 64;
+    const long trow = t0 + (lane & 15);
+    const int kofs = (lane >> 4) * 8;
+    f32x4 acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < NP; k0 += 32) {
+        uint2 q = make_uint2(0u, 0u);
+        if (trow < T) q = *(const uint2*)(xq + (size_t)trow * NP + k0 + kofs);
+        const i16x8 a = fp8x8_to_bf16x8(q);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const size_t wofs = (size_t)(c0 + c * 16 + (lane & 15)) * NP + k0 + kofs;
+            const i16x8 bhi = *(const i16x8*)(Whi + wofs);
+            const i16x8 blo = *(const i16x8*)(Wlo + wofs);
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                             __builtin_bit_cast(bf16x8, blo), acc[c], 0, 0, 0);
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                             __builtin_bit_cast(bf16x8, bhi), acc[c], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int col = c0 + c * 16 + (lane & 15);
+        const float bias = wc[col];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long t = t0 + (lane >> 4) * 4 + r;
+            if (t < T) b[(size_t)t * ldb + col] = acc[c][r] - bias;
+        }
+    }
+}
+
 /* Register-resident unmixing, b = x W~^T - W~ c with W~ = W_hi + W_lo (bf16):
  *   Q8 = false: x pre-split into bf16 hi and lo (written once per recording
  *     by the whitening kernel), b = x_hi W_hi + x_hi W_lo + x_lo W_hi
@@ -445,7 +490,8 @@ __global__ __launch_bounds__(512) void gx_f32_slab_kernel(const float* __restric
                                                           double* __restrict__ part, long T,
                                                           int NP, int ldb, long chunk) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = w * 64;
+    const int r0 = blockIdx.z * 512 + w * 64;      /* grid.z: groups of 512 g columns */
+    if (r0 >= ldb) return;                          /* no barriers below */
     const int k0 = blockIdx.y * 16 * CT;
     const long ts = (long)blockIdx.x * chunk;
     const long te = min(T, ts + chunk);
@@ -510,7 +556,8 @@ __global__ __launch_bounds__(512) void gx_bf16_packed_kernel(const uint16_t* __r
                                                              double* __restrict__ part, long nblk,
                                                              int NP, int ldb, long cblk) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = w * 64;
+    const int r0 = blockIdx.z * 512 + w * 64;      /* grid.z: groups of 512 g columns */
+    if (r0 >= ldb) return;                          /* no barriers below */
     const int k0 = blockIdx.y * 16 * CT;
     const long b0 = (long)blockIdx.x * cblk;
     const long b1 = min(nblk, b0 + cblk);
@@ -681,7 +728,10 @@ int launch_unmix_bf16x3(const float* x, const uint16_t* xbf, const uint16_t* Whi
 int launch_unmix_fp8(const uint8_t* xq, const uint16_t* Whi, const uint16_t* Wlo, const float* wc,
                      float* b, long T, int NP, int ldb, int cus, hipStream_t st) {
     if (NP % 32 || ldb % 64) return -1;
-    return launch_unmix_reg<true>(xq, nullptr, Whi, Wlo, wc, b, T, NP, ldb, cus, st);
+    if (launch_unmix_reg<true>(xq, nullptr, Whi, Wlo, wc, b, T, NP, ldb, cus, st) == 0) return 0;
+    dim3 grid((unsigned)((T + 63) / 64), ldb / 64);
+    hipLaunchKernelGGL(unmix_fp8_tiled_kernel, grid, dim3(256), 0, st, xq, Whi, Wlo, wc, b, T, NP, ldb);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 static long gx_chunk(long T, int nchunks, int mult) {
@@ -703,9 +753,9 @@ int gx_f32_chunks(long T, int NP, int ldb, int cus) {
 
 int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, int ldb,
                   int nchunks, hipStream_t st) {
-    if (NP % 64 || ldb % 64 || ldb > 512 || nchunks < 1) return -1;
+    if (NP % 64 || ldb % 64 || nchunks < 1) return -1;
     const int CT = gx_f32_ct(NP);
-    dim3 grid(nchunks, NP / (16 * CT)), block(ldb);
+    dim3 grid(nchunks, NP / (16 * CT), (ldb + 511) / 512), block(std::min(ldb, 512));
     const long chunk = gx_chunk(T, nchunks, 4);
     if (CT == 8)
         hipLaunchKernelGGL(gx_f32_slab_kernel<8>, grid, block, 0, st, g, x, part, T, NP, ldb, chunk);
@@ -716,19 +766,23 @@ int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, 
 
 static int gx_bf16_ct(int NP) { return NP % 128 == 0 ? 8 : 4; }
 
-int gx_bf16_chunks(long T, int NP, int cus) {
+int gx_bf16_chunks(long T, int NP, int ldb, int cus) {
+    /* at least one workgroup per CU of the budget where T allows, and at
+     * most 256 MB of partials */
     const long nblk = (T + 31) / 32;
-    const int slabs = NP / (16 * gx_bf16_ct(NP));
-    return (int)std::max(1L, std::min(nblk, (long)std::max(1, cus / slabs)));
+    const int slabs = NP / (16 * gx_bf16_ct(NP)) * ((ldb + 511) / 512);
+    const long by_mem = std::max(1L, (256L << 20) / ((long)ldb * NP * 8));
+    return (int)std::max(1L, std::min(std::min(nblk, by_mem), (long)std::max(1, cus / slabs)));
 }
 
 int launch_gx_bf16(const uint16_t* g8, const uint16_t* x8, double* part, long T, int NP, int ldb,
                    int nchunks, hipStream_t st) {
-    if (NP % 64 || ldb % 64 || ldb > 512 || nchunks < 1) return -1;
+    if (NP % 64 || ldb % 64 || nchunks < 1) return -1;
     const int CT = gx_bf16_ct(NP);
     const long nblk = (T + 31) / 32;
     const long cblk = (nblk + nchunks - 1) / nchunks;
-    dim3 grid(nchunks, NP / (16 * CT)), block(ldb);   /* ldb / 64 waves */
+    /* up to 8 waves of 64 g columns per workgroup, grid.z over the rest */
+    dim3 grid(nchunks, NP / (16 * CT), (ldb + 511) / 512), block(std::min(ldb, 512));
     if (CT == 8)
         hipLaunchKernelGGL(gx_bf16_packed_kernel<8>, grid, block, 0, st, g8, x8, part, nblk, NP, ldb, cblk);
     else

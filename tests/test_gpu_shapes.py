@@ -159,3 +159,85 @@ def test_batch_workload_each_recording_follows_oracle(built, precision):
              f"envelope {np.array2string(env, **fmt)}")
         assert rel[0] <= first_tol, r
         assert np.all(rel <= np.maximum(1e-6, 4.0 * env)), r
+
+
+# The split E-step: shapes whose models' 64-channel blocks do not fit one
+# 8-wave workgroup (M ceil(n/64) > 8) run three passes -- per-group channel
+# sums of q, the model posteriors, then the statistics -- with workgroups of
+# model groups x channel-block groups (the reference allows 1024 channels
+# and 16 models, config.h:15-16).  (n, T, M, m, precision): 8 models of 128
+# channels (2 model groups), 2 models of 512 (one model per group), 1024
+# channels in two channel groups (m = 1), 16 models of 64 channels, and 300
+# channels x 3 models in FP8 (the tiled FP8 unmixing kernel, NP = 320).
+@pytest.mark.parametrize("n,T,M,m,prec", [
+    (128, 12000, 8, 3, ca.PREC_FP32), (512, 5000, 2, 3, ca.PREC_BF16),
+    (1024, 3001, 1, 1, ca.PREC_FP32), (64, 8000, 16, 2, ca.PREC_BF16),
+    (300, 6000, 3, 4, ca.PREC_FP8)], ids=["n128M8", "n512M2", "n1024M1", "n64M16", "n300M3fp8"])
+def test_estep_stats_split(built, n, T, M, m, prec):
+    _say(f"[estep split] n={n} T={T} M={M} m={m} precision {prec}")
+    x = ca.synth_data(ca.synth_spec(n, T, seed=950 + n + M, cond_max=0.0))
+    st = _random_state(n, M, m, seed=951 + n + M)
+    with ca.Engine(ca.options(n, T, M, m, precision=prec)) as eng:
+        if prec == ca.PREC_FP8:
+            eng.load_raw(x)
+            eng.preprocess(1)
+            xin = eng.white().astype(np.float64)
+            ldetS = eng.sphere()[2]
+        elif n > 512:
+            # the raw mixtures as the data (statistics need no sphering; the
+            # oracle's Jacobi eigensolver is slow at 1024 channels)
+            xin, ldetS = x.astype(np.float64), 0.0
+            eng.load_white(xin, ldetS)
+        else:
+            xw, _, _, ldetS = po.preprocess(x, 1)
+            xin = xw.astype(np.float32).astype(np.float64)
+            eng.load_white(xin, ldetS)
+        eng.set_state(st)
+        gpu = eng.estep_stats()
+    ora = po.estep(xin, ldetS, st["A"], st["c"], st["alpha"], st["mu"], st["beta"], st["rho"],
+                   st["gm"], M, m)
+    if prec == ca.PREC_FP32:
+        # vsum of a model with 6 % of the samples (8 models): FP32 posteriors
+        _compare_stats(gpu, ora, n, m, M, vsum_tol=2e-6)
+    else:
+        _compare_stats(gpu, ora, n, m, M, tol=1e-3, ll_tol=1e-5, skip=("mu_denom",), vsum_tol=1e-4)
+
+
+def test_split_run_and_ltall_follow_oracle(built):
+    """8 models of 96 channels (split E-step), FP32 path: 5 EM iterations
+    against the FP64 oracle (first iteration within 1e-7, later ones within
+    max(1e-6, 4x the oracle's spread under float32-sized perturbations)),
+    then the per-sample model log-likelihoods of one more E-step (ltall)."""
+    n, T, M, m, iters = 96, 16000, 8, 3, 5
+    _say(f"[run split] n={n} T={T} M={M} m={m}, {iters} iterations")
+    x = ca.synth_data(ca.synth_spec(n, T, seed=960, nmix=M, block=T // (2 * M), alternate=1,
+                                    cond_max=100.0))
+    st0 = ca.synth_init(n, M, m, 961)
+    xw, _, _, ldetS = po.preprocess(x, 1)
+    xin = xw.astype(np.float32).astype(np.float64)
+    with ca.Engine(ca.options(n, T, M, m, max_iter=iters + 1)) as eng:
+        eng.load_white(xin, ldetS)
+        eng.set_state(st0)
+        eng.iterate(iters)
+        eng.sync()
+        glog = eng.log(iters)
+        gst = eng.get_state()
+        eng.record_ltall(True)
+        eng.iterate(1)
+        lt = eng.ltall()
+    cfg = po.default_config(n, T, M, m, maxiter=iters)
+    fin, lq = po.run(cfg, xin, ldetS, st0)
+    env = _envelope(cfg, xin, ldetS, st0, lq["ll"], 6e-8)
+    rel = np.abs(glog["ll"] - lq["ll"]) / np.abs(lq["ll"])
+    fmt = dict(formatter={"float_kind": lambda v: f"{v:.2e}"})
+    _say(f"  LL rel. error per iteration: {np.array2string(rel, **fmt)}; "
+         f"envelope {np.array2string(env, **fmt)}")
+    assert rel[0] <= 1e-7
+    assert np.all(rel <= np.maximum(1e-6, 4.0 * env))
+    # ltall of one E-step from the GPU's final state against the oracle's
+    cfg1 = po.default_config(n, T, M, m, maxiter=1)
+    st = {k: gst[k] for k in ("A", "c", "alpha", "mu", "beta", "rho", "gm")}
+    _, l1 = po.run(cfg1, xin, ldetS, st, want_ltall=True)
+    err = np.max(np.abs(lt - l1["ltall"]))
+    _say(f"  ltall max abs error {err:.2e} (|L| up to {np.abs(l1['ltall']).max():.1f})")
+    assert err <= 2e-6 * np.abs(l1["ltall"]).max()

@@ -11,7 +11,7 @@ gm, the lrate entering k and LL of k - 1) and compares one EM step
 parameter after k.  So each step of the whole trajectory is held to the
 per-step bar, whatever the trajectory does later.
 
-Bars per step: LL within 1e-6 relative (FP32 and BF16); decisions equal
+Bars per step: LL within 1e-6 relative (FP32; BF16 5e-6); decisions equal
 unless the oracle's own test is a near-tie (LL step, or min |D_ik - 1| of
 the Newton test kernels.cu:975-979, below 1e-4); parameters within the
 bars listed in PARAM_TOL (relative Frobenius norm per parameter; c against
@@ -35,7 +35,10 @@ PARAM_TOL = {
     32: dict(A=1e-6, c=1e-6, alpha=1e-6, mu=1e-5, beta=1e-6, rho=1e-6, gm=1e-7),
     16: dict(A=1e-4, c=1e-4, alpha=1e-5, mu=1e-2, beta=1e-3, rho=1e-5, gm=1e-4),
 }
-LL_TOL = 1e-6
+# LL of a step from an identical state: 1e-6 on the FP32 path; the BF16
+# path's unmixing rounds b to ~2^-16, which moves the E-step's LL by up to a
+# few 1e-6 (measured 1.4e-6 at 128 channels, 4 models)
+LL_TOL = {32: 1e-6, 16: 5e-6}
 TIE = 1e-4
 
 
@@ -126,9 +129,9 @@ def _report_and_check(res, precision):
     for k, step, marg in res["mism"]:
         _say(f"  decisions differ at step {k}: oracle LL step {step:.2e}, Newton-test margin {marg:.2e}")
     _say(f"  lrate after each step: max rel. error {res['lr'].max():.2e}")
-    assert np.all(res["ll"] <= LL_TOL), res["ll"]
+    assert np.all(res["ll"] <= LL_TOL[precision]), res["ll"]
     for k, step, marg in res["mism"]:
-        assert step <= 10 * LL_TOL or marg <= TIE, f"decision differs at step {k} without a near-tie"
+        assert step <= 10 * LL_TOL[precision] or marg <= TIE, f"decision differs at step {k} without a near-tie"
     # a step whose (near-tie) decision differs legitimately moves elsewhere
     ok = np.ones(len(res["ll"]), bool)
     for k, _, _ in res["mism"]:

@@ -49,7 +49,11 @@ def test_engine_reports_errors_without_touching_gpu(built):
     with pytest.raises(ca.EngineError):
         ca.Engine(ca.options(0, 100, 1, 3))            # n out of range
     with pytest.raises(ca.EngineError):
-        ca.Engine(ca.options(128, 100, 16, 3))         # M * ceil(n/64) > 8
+        ca.Engine(ca.options(1025, 100, 1, 3))         # beyond the reference's 1024 channels
+    with pytest.raises(ca.EngineError):
+        ca.Engine(ca.options(64, 100, 17, 3))          # beyond its 16 models
+    with pytest.raises(ca.EngineError):
+        ca.Engine(ca.options(64, 100, 2, 9))           # beyond 8 mixture components
 
 
 def test_synth_deterministic_and_unit_variance(built):
