@@ -6,10 +6,10 @@ command line program with the reference's parameter-file format.
 """
 from .engine import (BATCH_POLICIES, CLI_PATH, LIB_PATH, PARAM_NAMES, PREC_BF16, PREC_FP8, PREC_FP32, Batch,
                      Engine, EngineError,
-                     cond, hbm_probe, ll_adjoint, ll_forward, load_library, options, synth_data, synth_init,
+                     cond, hbm_probe, ll_adjoint, ll_forward, ll_forward_noalloc, load_library, options, synth_data, synth_init,
                      synth_spec)
 from .workloads import WORKLOADS, Workload, batch_workload
 
 __all__ = ["Engine", "EngineError", "options", "load_library", "synth_data", "synth_init",
            "synth_spec", "cond", "PREC_FP32", "PREC_BF16", "PREC_FP8", "LIB_PATH", "CLI_PATH", "WORKLOADS",
-           "Workload", "hbm_probe", "ll_forward", "ll_adjoint", "PARAM_NAMES", "Batch", "BATCH_POLICIES", "batch_workload"]
+           "Workload", "hbm_probe", "ll_forward", "ll_forward_noalloc", "ll_adjoint", "PARAM_NAMES", "Batch", "BATCH_POLICIES", "batch_workload"]

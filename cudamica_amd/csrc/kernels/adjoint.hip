@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "../../../include/amica_adjoint.h"
@@ -425,10 +426,49 @@ struct Workspace {
     }
 };
 
+/* The fixed device workspace of amica_ll_forward_noalloc (callers that
+ * cannot allocate: code under hipGraph capture, tight serving loops).  It
+ * holds, in doubles: W^-1, the LU and inverse scratch (M n n each), the
+ * per-(h, j, i) constants (M m n), W c (M NP) and log|det W| (M).  Limits:
+ * M n n <= kNoAllocNN (e.g. 256 channels x 1 model, 128 x 4), plus the
+ * forward's own M ceil(n/64) 64 <= 512, m <= 8.  1.6 MB per device, held
+ * until the process ends; every call on a device shares it. */
+constexpr size_t kNoAllocNN = 65536;
+constexpr size_t kNoAllocNM = 4096;        /* 8 components x 512 padded channels */
+constexpr size_t kNoAllocNP = 512;
+constexpr size_t kNoAllocM = 8;
+constexpr size_t kNoAllocDoubles = 3 * kNoAllocNN + kNoAllocNM + kNoAllocNP + kNoAllocM;
+
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+/* The device address of `scratch` on the current device, looked up once per
+ * device (each GPU holds its own copy of the array at its own address; the
+ * array's name in host code is only a handle for the lookup). */
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return report_error(AMICA_E_DEVICE, "no HIP device");
+    if (dev < 0 || dev >= kMaxDevices) return report_error(AMICA_E_DEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return report_error(AMICA_E_DEVICE, (std::string("hipGetSymbolAddress(scratch): ") +
+                                                 hipGetErrorString(e)).c_str());
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 int ll_call(int real_bits, const void* x, long T, int n, int M, int m, const double* W,
             const double* c, const double* alpha, const double* mu, const double* beta,
             const double* rho, const double* gm, double ldetS, const double* dll, double* ll,
-            const AdjOut* out, void* stream) {
+            const AdjOut* out, void* stream, bool noalloc = false) {
     const bool adj = out != nullptr;
     if (real_bits != 32 && real_bits != 64) return report_error(AMICA_E_INVALID, "real_bits must be 32 or 64");
     if (!x || !W || !c || !alpha || !mu || !beta || !rho || !gm || (adj && !dll))
@@ -461,10 +501,24 @@ int ll_call(int real_bits, const void* x, long T, int n, int M, int m, const dou
     a.pstride = ((size_t)M * NP * pair_stride(m) + M + 31) & ~(size_t)31;
     Workspace ws;
     const size_t nn = (size_t)n * n;
-    if (!ws.alloc((void**)&a.Winv, M * nn * 8) || !ws.alloc((void**)&a.ldet, M * 8) ||
-        !ws.alloc((void**)&a.cq, (size_t)M * m * n * 8) || !ws.alloc((void**)&a.wc, (size_t)M * NP * 8) ||
-        !ws.alloc((void**)&a.lu, M * nn * 8) || !ws.alloc((void**)&a.X, M * nn * 8))
+    if (noalloc) {
+        /* forward only, workspace carved from the static device array */
+        if ((size_t)M * nn > kNoAllocNN)
+            return report_error(AMICA_E_INVALID, "M * n * n exceeds the allocation-free workspace (65536)");
+        double* sp = nullptr;
+        const int rc = noalloc_scratch(&sp);
+        if (rc) return rc;
+        a.Winv = sp;
+        a.lu = sp + kNoAllocNN;
+        a.X = sp + 2 * kNoAllocNN;
+        a.cq = sp + 3 * kNoAllocNN;
+        a.wc = a.cq + kNoAllocNM;
+        a.ldet = a.wc + kNoAllocNP;
+    } else if (!ws.alloc((void**)&a.Winv, M * nn * 8) || !ws.alloc((void**)&a.ldet, M * 8) ||
+               !ws.alloc((void**)&a.cq, (size_t)M * m * n * 8) || !ws.alloc((void**)&a.wc, (size_t)M * NP * 8) ||
+               !ws.alloc((void**)&a.lu, M * nn * 8) || !ws.alloc((void**)&a.X, M * nn * 8)) {
         return report_error(AMICA_E_NOMEM, "workspace allocation failed");
+    }
     const int gchunks = (int)std::max(1L, std::min((long)cus, (T + 255) / 256));
     double* gxpart = nullptr;
     if (adj) {
@@ -496,6 +550,7 @@ int ll_call(int real_bits, const void* x, long T, int n, int M, int m, const dou
         hipLaunchKernelGGL(adj_finish_kernel, dim3(M), dim3(256), (size_t)n * 8, st, a, nb, gxpart, gc, *out);
         if (hipGetLastError() != hipSuccess) return report_error(AMICA_E_DEVICE, "adjoint launch failed");
     }
+    if (noalloc) return 0;     /* asynchronous: the caller synchronises `stream` */
     const hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return report_error(AMICA_E_DEVICE, hipGetErrorString(e));
     return 0;
@@ -511,6 +566,14 @@ extern "C" int amica_ll_forward(int real_bits, const void* x, long T, int n, int
                                 const double* gm, double ldetS, double* ll, void* stream) {
     return amica::ll_call(real_bits, x, T, n, M, m, W, c, alpha, mu, beta, rho, gm, ldetS, nullptr,
                           ll, nullptr, stream);
+}
+
+extern "C" int amica_ll_forward_noalloc(int real_bits, const void* x, long T, int n, int M, int m,
+                                        const double* W, const double* c, const double* alpha,
+                                        const double* mu, const double* beta, const double* rho,
+                                        const double* gm, double ldetS, double* ll, void* stream) {
+    return amica::ll_call(real_bits, x, T, n, M, m, W, c, alpha, mu, beta, rho, gm, ldetS, nullptr,
+                          ll, nullptr, stream, true);
 }
 
 extern "C" int amica_ll_adjoint(int real_bits, const void* x, long T, int n, int M, int m,

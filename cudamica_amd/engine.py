@@ -91,6 +91,9 @@ _SIGNATURES = [
     ("amica_batch_sync", ctypes.c_int, [_V]),
     ("amica_ll_forward", ctypes.c_int, [ctypes.c_int, _V, ctypes.c_long, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int] + [_V] * 7 + [ctypes.c_double, _V, _V]),
+    ("amica_ll_forward_noalloc", ctypes.c_int, [ctypes.c_int, _V, ctypes.c_long, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int] + [_V] * 7
+     + [ctypes.c_double, _V, _V]),
     ("amica_ll_adjoint", ctypes.c_int, [ctypes.c_int, _V, ctypes.c_long, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int] + [_V] * 7 + [ctypes.c_double] + [_V] * 10),
 ]
@@ -413,6 +416,14 @@ def ll_forward(real_bits, x, T, n, M, m, params, ldetS, ll, stream=0):
     L = load_library()
     _check(L.amica_ll_forward(real_bits, x, T, n, M, m, *[params[k] for k in PARAM_NAMES],
                               ldetS, ll, stream), "amica_ll_forward")
+
+
+def ll_forward_noalloc(real_bits, x, T, n, M, m, params, ldetS, ll, stream=0):
+    """amica_ll_forward_noalloc: as ll_forward with the library's static
+    workspace (allocates nothing, does not synchronise `stream`)."""
+    L = load_library()
+    _check(L.amica_ll_forward_noalloc(real_bits, x, T, n, M, m, *[params[k] for k in PARAM_NAMES],
+                                      ldetS, ll, stream), "amica_ll_forward_noalloc")
 
 
 def ll_adjoint(real_bits, x, T, n, M, m, params, ldetS, dll, grads, ll=0, stream=0):

@@ -53,6 +53,20 @@ int amica_ll_forward(int real_bits, const void* x, long T, int n, int M, int m,
                      const double* beta, const double* rho, const double* gm, double ldetS,
                      double* ll, void* stream);
 
+/* amica_ll_forward without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as amica_ll_forward; its workspace is a static
+ * device array of 1.6 MB per GPU (held until the process ends), so it also
+ * requires M * n * n <= 65536 and rejects larger models with
+ * AMICA_E_INVALID.  The call only enqueues work on `stream` and returns:
+ * the caller synchronises it.  Calls on one GPU share the workspace, so
+ * calls that overlap on one GPU (other streams, other host threads) race. */
+int amica_ll_forward_noalloc(int real_bits, const void* x, long T, int n, int M, int m,
+                             const double* W, const double* c, const double* alpha,
+                             const double* mu, const double* beta, const double* rho,
+                             const double* gm, double ldetS, double* ll, void* stream);
+
 int amica_ll_adjoint(int real_bits, const void* x, long T, int n, int M, int m,
                      const double* W, const double* c, const double* alpha, const double* mu,
                      const double* beta, const double* rho, const double* gm, double ldetS,
