@@ -282,7 +282,7 @@ def bench_batch(args):
     prec = PRECISIONS[args.precision or "fp32"]
     ws = ca.batch_workload(precision=prec)
     t0 = time.perf_counter()
-    opts = [w.options(max_iter=args.warmup + args.steps + 4) for w in ws]
+    opts = [w.options(max_iter=args.warmup + args.steps + 8 + 3 * len(ca.BATCH_POLICIES)) for w in ws]
     b = ca.Batch(opts)
     for w, e in zip(ws, b.engines):
         e.load_raw(ca.synth_data(w.spec()))
@@ -301,7 +301,19 @@ def bench_batch(args):
     b.sync()
     dt = time.perf_counter() - t0
     lls = [float(e.log(args.warmup + args.steps)["ll"][-1]) for e in b.engines]
+    # each recording alone on the GPU (its own one-iteration graph), the
+    # batched iteration's yardstick: the sum of the solo iteration times
+    solo = []
+    for e in b.engines:
+        e.iterate(1, graph=True)
+        e.sync()
+        t1 = time.perf_counter()
+        e.iterate(2, graph=True)
+        e.sync()
+        solo.append((time.perf_counter() - t1) / 2)
     b.close()
+    log(f"batched iteration {dt * 1e3 / args.steps:.2f} ms; sum of the recordings' solo iterations "
+        f"{sum(solo) * 1e3:.2f} ms")
     out = {
         "metric": "batched EM iters/s (32 recordings per iteration, one hipGraph)",
         "value": args.steps / dt, "unit": "batched EM iters/s", "n_gpus": 1,
@@ -313,6 +325,8 @@ def bench_batch(args):
         "tuner_ms": dict(zip(ca.BATCH_POLICIES, [float(t) for t in ms])),
         "policy": ca.BATCH_POLICIES[p],
         "recording_iters_per_s": len(ws) * args.steps / dt,
+        "solo_ms_sum": sum(solo) * 1e3,
+        "solo_ms": [t * 1e3 for t in solo],
         "ll_finite": bool(np.all(np.isfinite(lls))),
     }
     print(json.dumps(out), flush=True)
@@ -332,6 +346,8 @@ def main():
                     help="skip the 50-iteration error report against the FP64 oracle")
     ap.add_argument("--no-ll-check", action="store_true",
                     help="skip the 50-iteration LL comparison with the FP32 path")
+    ap.add_argument("--cu-budget", type=int, default=0, metavar="CUS",
+                    help="size the grids for CUS compute units (occupancy studies; 0: all)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a memset + copy of MIB MiB first (counter calibration)")
     args = ap.parse_args()
@@ -358,6 +374,8 @@ def main():
     log(f"data {T}x{n} generated in {time.perf_counter() - t0:.1f} s")
 
     eng = ca.Engine(w.options(precision=prec, max_iter=args.warmup + args.steps + 1))
+    if args.cu_budget:
+        eng.set_cu_budget(args.cu_budget)
     eng.load_raw(x)
     t0 = time.perf_counter()
     eng.preprocess(w.do_sphere)
@@ -401,7 +419,8 @@ def main():
         "dtype": DTYPE[prec],
         "data": "synthetic",
         "config": {"workload": w.name, "n": n, "T": T, "M": M, "m": m,
-                   "precision": PREC_NAME[prec], "parallelism": "single-gpu"},
+                   "precision": PREC_NAME[prec], "parallelism": "single-gpu",
+                   **({"cu_budget": args.cu_budget} if args.cu_budget else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(w.name, prec),
                      "kernel": "estep_kernel", "bytes_per_launch": nbytes,

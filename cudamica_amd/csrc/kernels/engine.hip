@@ -207,11 +207,24 @@ struct amica_engine {
         return a;
     }
 
+    /* W = A^-1, log|det W| and the GEMM copies of W and the E-step constants */
+    int enqueue_prepare() {
+        LCHK(launch_prepare(margs(), st));
+        return 0;
+    }
+
+    /* the per-recording E-step partials (nparts slots) and statistic-GEMM
+     * partials (nch slots) into the canonical statistics */
+    int enqueue_reduce(int nparts, int nch) {
+        LCHK(launch_reduce(part, nparts, pstride, gxpart, nch, raw, gxtot, sumtmp, margs(), st));
+        return 0;
+    }
+
     /* prepare, unmixing GEMM, E-step, statistic GEMM, reductions */
     int enqueue_estep(bool with_timing) {
-        const MstepArgs ma = margs();
         if (with_timing) HIPCHK(hipEventRecord(ev[0], st));
-        LCHK(launch_prepare(ma, st));
+        int rc = enqueue_prepare();
+        if (rc) return rc;
         if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
         if (bf16)
         {
@@ -228,14 +241,26 @@ struct amica_engine {
         else
             LCHK(launch_gx_f32((const float*)g, xw, gxpart, o.T, NP, ldb, nchunks, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[4], st));
-        LCHK(launch_reduce(part, nblocks, pstride, gxpart, nchunks, raw, gxtot, sumtmp, ma, st));
+        return enqueue_reduce(nblocks, nchunks);
+    }
+
+    /* The batch's launches over many recordings cover the BF16 single-model
+     * path: unmix_lds_kernel shapes (NP <= 256), estep_m1_kernel, the packed
+     * statistic GEMM. */
+    bool groupable() const {
+        return bf16 && !fp8 && o.M == 1 && !split && !ltall_on && NP <= 256;
+    }
+
+    int enqueue_mstep() {
+        LCHK(launch_mstep(margs(), st));
         return 0;
     }
 
     int enqueue_iteration(bool with_timing) {
         int rc = enqueue_estep(with_timing);
         if (rc) return rc;
-        LCHK(launch_mstep(margs(), st));
+        rc = enqueue_mstep();
+        if (rc) return rc;
         if (with_timing) HIPCHK(hipEventRecord(ev[5], st));
         return 0;
     }
@@ -441,6 +466,14 @@ int amica_engine_destroy(amica_engine* e) {
 }
 
 int amica_engine_cu_count(amica_engine* e) { return e ? e->cus : 0; }
+
+int amica_engine_set_cu_budget(amica_engine* e, int cus) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    e->size_grids(cus > 0 ? cus : e->cus);
+    return 0;
+}
 
 int amica_engine_load_raw(amica_engine* e, const float* x, int x_on_device) {
     if (!e || !x) return fail(AMICA_E_INVALID, "null argument");
@@ -834,20 +867,182 @@ struct amica_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int mode = 0;                       /* launch mode of the current policy */
+    long item_samples = 0;              /* mode 2: samples per work item (0: each recording's own partition) */
+    hipEvent_t fork2 = nullptr;
+    std::vector<hipEvent_t> join2;
+
+    /* mode 2: the recordings whose heavy kernels share launches (same NP and
+     * m), with their per-recording operands and work lists on the device */
+    struct Group {
+        int NP = 0, ldb = 0, m = 0;
+        std::vector<int> recs;
+        void* dmem = nullptr;
+        UnmixArgs* ua = nullptr;
+        EstepArgs* ea = nullptr;
+        GxArgs* ga = nullptr;
+        BatchItem *iu = nullptr, *ie = nullptr, *ig = nullptr;
+        int nu = 0, ne = 0, ng = 0;
+    };
+    std::vector<Group> groups;
+    std::vector<int> gr_nparts, gr_nch;   /* per recording: E-step and GEMM partial slots used */
+
+    void drop_groups() {
+        for (Group& g : groups)
+            if (g.dmem) (void)hipFree(g.dmem);
+        groups.clear();
+    }
 
     void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
+        drop_groups();
+    }
+
+    /* Work lists of the grouped launches: every recording's samples in
+     * chunks of item_samples (at least T / its partial slots, whole 32-sample
+     * blocks), the unmixing in the same ranges as 16-sample tiles, the
+     * statistic GEMM as 32-sample blocks. */
+    int build_groups() {
+        drop_groups();
+        const size_t R = eng.size();
+        gr_nparts.assign(R, 0);
+        gr_nch.assign(R, 0);
+        for (size_t r = 0; r < R; ++r) {
+            const amica_engine* e = eng[r];
+            if (!e->groupable()) continue;
+            Group* g = nullptr;
+            for (Group& q : groups)
+                if (q.NP == e->NP && q.m == e->o.m) g = &q;
+            if (!g) {
+                groups.emplace_back();
+                g = &groups.back();
+                g->NP = e->NP;
+                g->ldb = e->ldb;
+                g->m = e->o.m;
+            }
+            g->recs.push_back((int)r);
+        }
+        for (Group& g : groups) {
+            std::vector<UnmixArgs> ua;
+            std::vector<EstepArgs> ea;
+            std::vector<GxArgs> ga;
+            std::vector<BatchItem> iu, ie, ig;
+            for (size_t k = 0; k < g.recs.size(); ++k) {
+                const int r = g.recs[k];
+                amica_engine* e = eng[r];
+                const long T = e->o.T;
+                /* item_samples 0: the recording's own grid partition, so that
+                 * every partial sum covers the samples it covers in the
+                 * recording's own launches (identical results) */
+                long chunk = item_samples > 0
+                                 ? std::max(item_samples, (T + e->nblocks_cap - 1) / e->nblocks_cap)
+                                 : e->chunk;
+                chunk = (chunk + 31) / 32 * 32;
+                const int nparts = (int)((T + chunk - 1) / chunk);
+                for (int i = 0; i < nparts; ++i) {
+                    const long t0 = (long)i * chunk, t1 = std::min(T, t0 + chunk);
+                    ie.push_back(BatchItem{(int)k, i, t0, t1});
+                    iu.push_back(BatchItem{(int)k, 0, t0 / 16, (t1 + 15) / 16});
+                }
+                const long nblk = (T + 31) / 32;
+                const long per = item_samples > 0
+                                     ? std::max(chunk / 32, (nblk + e->nchunks_cap - 1) / e->nchunks_cap)
+                                     : (nblk + e->nchunks - 1) / e->nchunks;
+                const int nch = (int)((nblk + per - 1) / per);
+                for (int i = 0; i < nch; ++i)
+                    ig.push_back(BatchItem{(int)k, i, (long)i * per, std::min(nblk, (long)(i + 1) * per)});
+                gr_nparts[r] = nparts;
+                gr_nch[r] = nch;
+                EstepArgs a = e->eargs(nullptr);
+                a.chunk = chunk;
+                ea.push_back(a);
+                ua.push_back(UnmixArgs{e->xbf, e->xbf + (size_t)T * e->NP, e->Whi, e->Wlo, e->wc, e->b, T, e->ldb});
+                ga.push_back(GxArgs{(const uint16_t*)e->g, e->x8, e->gxpart});
+            }
+            auto bytes = [](size_t n, size_t sz) { return (n * sz + 255) / 256 * 256; };
+            const size_t b_ua = bytes(ua.size(), sizeof(UnmixArgs)), b_ea = bytes(ea.size(), sizeof(EstepArgs)),
+                         b_ga = bytes(ga.size(), sizeof(GxArgs)), b_iu = bytes(iu.size(), sizeof(BatchItem)),
+                         b_ie = bytes(ie.size(), sizeof(BatchItem)), b_ig = bytes(ig.size(), sizeof(BatchItem));
+            HIPCHK(hipMalloc(&g.dmem, b_ua + b_ea + b_ga + b_iu + b_ie + b_ig));
+            char* p = (char*)g.dmem;
+            g.ua = (UnmixArgs*)p; p += b_ua;
+            g.ea = (EstepArgs*)p; p += b_ea;
+            g.ga = (GxArgs*)p; p += b_ga;
+            g.iu = (BatchItem*)p; p += b_iu;
+            g.ie = (BatchItem*)p; p += b_ie;
+            g.ig = (BatchItem*)p;
+            HIPCHK(hipMemcpy(g.ua, ua.data(), ua.size() * sizeof(UnmixArgs), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(g.ea, ea.data(), ea.size() * sizeof(EstepArgs), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(g.ga, ga.data(), ga.size() * sizeof(GxArgs), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(g.iu, iu.data(), iu.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(g.ie, ie.data(), ie.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(g.ig, ig.data(), ig.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+            g.nu = (int)iu.size();
+            g.ne = (int)ie.size();
+            g.ng = (int)ig.size();
+        }
+        return 0;
+    }
+
+    /* mode 2: per recording W = A^-1 and the constants (branches); on the
+     * capture's origin stream, per group one launch for each of the
+     * unmixing GEMM, the E-step and the statistic GEMM over all its
+     * recordings; then per recording the reductions and the M-step
+     * (branches).  Recordings outside the groups run their whole iteration
+     * on their own branch.  The groups' launches stay on the origin
+     * stream: streams created for them and first used inside the capture
+     * ended the process with a segmentation fault in the first grouped
+     * iteration. */
+    int enqueue_grouped() {
+        int rc = hipEventRecord(fork, cap) == hipSuccess ? 0 : AMICA_E_DEVICE;
+        for (size_t r = 0; r < eng.size() && rc == 0; ++r) {
+            amica_engine* e = eng[r];
+            if (hipStreamWaitEvent(e->st, fork, 0) != hipSuccess) { rc = AMICA_E_DEVICE; break; }
+            rc = e->groupable() ? e->enqueue_prepare() : e->enqueue_iteration(false);
+            if (rc) break;
+            if (hipEventRecord(join[r], e->st) != hipSuccess) { rc = AMICA_E_DEVICE; break; }
+            if (e->groupable() && hipStreamWaitEvent(cap, join[r], 0) != hipSuccess) rc = AMICA_E_DEVICE;
+        }
+        for (const Group& g : groups) {
+            if (rc) break;
+            if (launch_unmix_batch(g.ua, g.iu, g.nu, g.NP, g.ldb, cap) ||
+                launch_estep_m1_batch(g.ea, g.ie, g.ne, g.m, g.NP, cap) ||
+                launch_gx_bf16_batch(g.ga, g.ig, g.ng, g.NP, g.ldb, cap))
+                rc = fail(AMICA_E_DEVICE, std::string("grouped launch failed: ") + hipGetErrorString(hipGetLastError()));
+        }
+        if (!rc && hipEventRecord(fork2, cap) != hipSuccess) rc = AMICA_E_DEVICE;
+        for (size_t r = 0; r < eng.size() && rc == 0; ++r) {
+            amica_engine* e = eng[r];
+            if (!e->groupable()) {
+                if (hipStreamWaitEvent(cap, join[r], 0) != hipSuccess) rc = AMICA_E_DEVICE;
+                continue;
+            }
+            if (hipStreamWaitEvent(e->st, fork2, 0) != hipSuccess) { rc = AMICA_E_DEVICE; break; }
+            rc = e->enqueue_reduce(gr_nparts[r], gr_nch[r]);
+            if (!rc) rc = e->enqueue_mstep();
+            if (rc) break;
+            if (hipEventRecord(join2[r], e->st) != hipSuccess ||
+                hipStreamWaitEvent(cap, join2[r], 0) != hipSuccess)
+                rc = AMICA_E_DEVICE;
+        }
+        return rc;
     }
 
     /* one iteration of every recording: fork from `cap`, each recording on
-     * its own stream, join back into `cap` */
+     * its own stream (mode 0), or the grouped launches (mode 2), join back
+     * into `cap` */
     int capture() {
+        if (mode == 2) {
+            const int brc = build_groups();
+            if (brc) return brc;
+        }
         HIPCHK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        int rc = hipEventRecord(fork, cap) == hipSuccess ? 0 : AMICA_E_DEVICE;
-        for (size_t r = 0; r < eng.size() && rc == 0; ++r) {
+        int rc = 0;
+        if (mode == 2) rc = enqueue_grouped();
+        else rc = hipEventRecord(fork, cap) == hipSuccess ? 0 : AMICA_E_DEVICE;
+        for (size_t r = 0; mode != 2 && r < eng.size() && rc == 0; ++r) {
             if (hipStreamWaitEvent(eng[r]->st, fork, 0) != hipSuccess) { rc = AMICA_E_DEVICE; break; }
             rc = eng[r]->enqueue_iteration(false);
             if (rc) break;
@@ -877,7 +1072,10 @@ int amica_batch_destroy(amica_batch* b) {
     b->drop_graph();
     for (hipEvent_t ev : b->join)
         if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : b->join2)
+        if (ev) (void)hipEventDestroy(ev);
     if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->fork2) (void)hipEventDestroy(b->fork2);
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->cap) (void)hipStreamDestroy(b->cap);
     for (amica_engine* e : b->eng)
@@ -905,15 +1103,18 @@ int amica_batch_create(const amica_options* opts, int nrec, amica_batch** out) {
     }
     b->cus = b->eng[0]->cus;
     b->join.assign(nrec, nullptr);
+    b->join2.assign(nrec, nullptr);
     if (hipSetDevice(b->device) != hipSuccess ||
         hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
         amica_batch_destroy(b);
         return fail(AMICA_E_DEVICE, "batch stream or event creation failed");
     }
     for (int r = 0; r < nrec; ++r)
-        if (hipEventCreateWithFlags(&b->join[r], hipEventDisableTiming) != hipSuccess) {
+        if (hipEventCreateWithFlags(&b->join[r], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->join2[r], hipEventDisableTiming) != hipSuccess) {
             amica_batch_destroy(b);
             return fail(AMICA_E_DEVICE, "batch event creation failed");
         }
@@ -944,6 +1145,15 @@ int amica_batch_set_budgets(amica_batch* b, const int* cu_budget) {
 int amica_batch_set_policy(amica_batch* b, int p) {
     if (!b) return fail(AMICA_E_INVALID, "null batch");
     if (p < 0 || p >= AMICA_BATCH_POLICIES) return fail(AMICA_E_INVALID, "no such policy");
+    if (p >= 8) {
+        /* grouped launches over all recordings; work items of each recording's
+         * own partition, or of 512, 1024 or 2048 samples */
+        static const long kItem[4] = {0, 512, 1024, 2048};
+        b->drop_graph();
+        b->mode = 2;
+        b->item_samples = kItem[p - 8];
+        return amica_batch_set_budgets(b, nullptr);
+    }
     b->mode = p / 4;
     p %= 4;
     const int R = (int)b->eng.size(), cus = b->cus;

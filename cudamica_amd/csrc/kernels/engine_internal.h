@@ -47,6 +47,33 @@ constexpr int kEstepRowPad = 16;
  * sqrt(beta) and -sqrt(beta) mu, so that y is formed in FP64. */
 enum { kCst64Sb = 0, kCst64Nsbmu = 1, kNumCst64 = 2 };
 
+/* A work item of a launch over many recordings: recording `rec`, the range
+ * [t0, t1) of its samples (or of its 16- or 32-sample tiles), written into
+ * its partial slot `slot`. */
+struct BatchItem {
+    int rec;
+    int slot;
+    long t0;
+    long t1;
+};
+
+/* Per-recording operands of the batched unmixing and statistic GEMMs. */
+struct UnmixArgs {
+    const void* xa;         /* bf16 hi part (or FP8 q), [T][NP] */
+    const uint16_t* xlo;    /* bf16 lo part, or null (FP8) */
+    const uint16_t* Whi;
+    const uint16_t* Wlo;
+    const float* wc;
+    float* b;
+    long T;
+    int ldb;
+};
+struct GxArgs {
+    const uint16_t* g8;
+    const uint16_t* x8;
+    double* part;
+};
+
 /* E-step launches (estep.hip): fused, or the split passes */
 enum { kEstepFused = 0, kEstepLsum = 1, kEstepStats = 2 };
 
@@ -78,6 +105,10 @@ struct EstepArgs {
  * models per group, channel blocks per group, channel groups, groups. */
 void estep_split_shape(int M, int CW, int* mg, int* cwg, int* ncg, int* ngroups);
 int launch_estep(const EstepArgs& a, int m, int nblocks, hipStream_t st);
+/* Single-model E-steps of many recordings (BF16 path, same NP) in one
+ * launch, one workgroup per work item (device arrays). */
+int launch_estep_m1_batch(const EstepArgs* args, const BatchItem* items, int nitems, int m, int NP,
+                          hipStream_t st);
 /* Allows the E-step variants for m components `lds_bytes` of dynamic LDS. */
 int estep_configure(int m, size_t lds_bytes);
 /* Dynamic LDS of one E-step workgroup of `threads` threads. */
@@ -97,6 +128,14 @@ int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double
 int launch_unmix_bf16x3(const float* x, const uint16_t* xbf, const uint16_t* Whi,
                         const uint16_t* Wlo, const float* wc, float* b, long T, int NP, int ldb,
                         int cus, hipStream_t st);
+/* Split-BF16 unmixing of many recordings (same NP and ldb, NP <= 256) in one
+ * launch, one workgroup per work item of 16-sample tiles. */
+int launch_unmix_batch(const UnmixArgs* args, const BatchItem* items, int nitems, int NP, int ldb,
+                       hipStream_t st);
+/* Packed BF16 statistic GEMM of many recordings (same NP and ldb) in one
+ * launch, one workgroup column per work item of 32-sample blocks. */
+int launch_gx_bf16_batch(const GxArgs* args, const BatchItem* items, int nitems, int NP, int ldb,
+                         hipStream_t st);
 /* FP8 path: b = q W_hi + q W_lo with q the FP8 e4m3 data [T][NP] and W~ the
  * split of W diag(scale) (NP <= 256). */
 int launch_unmix_fp8(const uint8_t* xq, const uint16_t* Whi, const uint16_t* Wlo, const float* wc,

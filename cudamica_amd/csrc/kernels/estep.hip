@@ -513,7 +513,8 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
  * model constant is added by the reduction); vsum is the sample count.
  * Not used when per-sample L is recorded (ltall). */
 template <int MJ, bool B64>
-__global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
+__device__ __forceinline__ void estep_m1_body(const EstepArgs& a, const long t0, const long t1,
+                                              const int slot) {
     constexpr int NSJ = kStatsPerJ;
     constexpr int NS = NSJ * MJ + kStatsPerI;
     constexpr int G = 8;
@@ -550,12 +551,10 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
     }
     float qacc = 0.f;
     double qtot = 0.0;
-    double* const base = a.part + (size_t)blockIdx.x * a.pstride;
+    double* const base = a.part + (size_t)slot * a.pstride;
     double* const l3 = base + (size_t)i * NS;
     bool l3_started = false;
 
-    const long t0 = (long)blockIdx.x * a.chunk;
-    const long t1 = min(a.T, t0 + a.chunk);
     const size_t bcol = (size_t)i;
     uint4* const g8 = (uint4*)a.g;
 
@@ -709,6 +708,23 @@ __global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
     }
 }
 
+template <int MJ, bool B64>
+__global__ __launch_bounds__(512) void estep_m1_kernel(EstepArgs a) {
+    const long t0 = (long)blockIdx.x * a.chunk;
+    estep_m1_body<MJ, B64>(a, t0, min(a.T, t0 + a.chunk), (int)blockIdx.x);
+}
+
+/* Many recordings in one launch: workgroup b runs work item b, a range of
+ * samples of one recording (args[item.rec]) into that recording's partial
+ * slot item.slot.  Every recording of a launch has the same NP (blockDim). */
+template <int MJ, bool B64>
+__global__ __launch_bounds__(512) void estep_m1_batch_kernel(const EstepArgs* __restrict__ args,
+                                                             const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const EstepArgs a = args[it.rec];
+    estep_m1_body<MJ, B64>(a, it.t0, it.t1, it.slot);
+}
+
 /* The split launch's middle pass: per sample, the model log-likelihoods
  * L_h = (log gm_h + ldet_h + ldetS) + sum of q over all channels (the
  * channel-group sums of the first pass), their log-sum-exp (the sample's
@@ -794,6 +810,12 @@ void estep_split_shape(int M, int CW, int* mg, int* cwg, int* ncg, int* ngroups)
 }
 
 int estep_configure(int MJ, size_t lds_bytes) {
+    /* one attribute per kernel for the whole process: keep the largest that
+     * any engine asked for (engines of other shapes launch with less) */
+    static size_t granted[AMICA_MAX_MJ + 1] = {0};
+    if (MJ < 1 || MJ > AMICA_MAX_MJ) return -1;
+    if (lds_bytes <= granted[MJ]) return 0;
+    granted[MJ] = lds_bytes;
     hipError_t r = hipSuccess;
 #define AMICA_ESTEP_ATTR(J)                                                                       \
     case J:                                                                                       \
@@ -807,6 +829,9 @@ int estep_configure(int MJ, size_t lds_bytes) {
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
         if (r == hipSuccess)                                                                      \
             r = hipFuncSetAttribute((const void*)estep_kernel<J, false, kEstepStats>,            \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
+        if (r == hipSuccess)                                                                      \
+            r = hipFuncSetAttribute((const void*)estep_m1_batch_kernel<J, false>,                \
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);  \
         if (r == hipSuccess)                                                                      \
             r = hipFuncSetAttribute((const void*)estep_m1_kernel<J, true>,                       \
@@ -899,6 +924,30 @@ static int launch_estep_split(const EstepArgs& a, int MJ, int nblocks, hipStream
         default: return -1;
     }
 #undef AMICA_ESTEP_SPLIT
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_estep_m1_batch(const EstepArgs* args, const BatchItem* items, int nitems, int MJ, int NP,
+                          hipStream_t st) {
+    if (nitems < 1 || NP % 64 || NP > 64 * kMaxEstepWaves) return -1;
+    const dim3 grid(nitems), block(NP);
+    const size_t shm = estep_lds_bytes(MJ, NP);
+#define AMICA_ESTEP_BATCH(J)                                                                        \
+    case J:                                                                                         \
+        hipLaunchKernelGGL((estep_m1_batch_kernel<J, false>), grid, block, shm, st, args, items); \
+        break;
+    switch (MJ) {
+        AMICA_ESTEP_BATCH(1)
+        AMICA_ESTEP_BATCH(2)
+        AMICA_ESTEP_BATCH(3)
+        AMICA_ESTEP_BATCH(4)
+        AMICA_ESTEP_BATCH(5)
+        AMICA_ESTEP_BATCH(6)
+        AMICA_ESTEP_BATCH(7)
+        AMICA_ESTEP_BATCH(8)
+        default: return -1;
+    }
+#undef AMICA_ESTEP_BATCH
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -243,14 +243,17 @@ __global__ __launch_bounds__(256) void unmix_fp8_tiled_kernel(const uint8_t* __r
  * sample, per column group) + b (4 ldb bytes per sample). */
 constexpr int kUnmixStages = 4;
 
+/* The workgroup's tiles: tile = first + j stride for j < ntl; column group
+ * cgroup (of 128 CT columns). */
 template <int CT, int KS, bool Q8>
-__global__ __launch_bounds__(512) void unmix_lds_kernel(const void* __restrict__ xa,
-                                                         const uint16_t* __restrict__ xlo,
-                                                         const uint16_t* __restrict__ Whi,
-                                                         const uint16_t* __restrict__ Wlo,
-                                                         const float* __restrict__ wc,
-                                                         float* __restrict__ b, long T, int ldb,
-                                                         long ntiles) {
+__device__ __forceinline__ void unmix_lds_body(const void* __restrict__ xa,
+                                               const uint16_t* __restrict__ xlo,
+                                               const uint16_t* __restrict__ Whi,
+                                               const uint16_t* __restrict__ Wlo,
+                                               const float* __restrict__ wc,
+                                               float* __restrict__ b, const long T, const int ldb,
+                                               const long first, const long stride, const long ntl,
+                                               const int cgroup) {
     constexpr int NP = 32 * KS;
     constexpr int PARTS = Q8 ? 1 : 2;              /* fp8 q, or bf16 hi and lo */
     constexpr int ROWB = Q8 ? NP : 2 * NP;         /* bytes of one sample row of one part */
@@ -265,7 +268,7 @@ __global__ __launch_bounds__(512) void unmix_lds_kernel(const void* __restrict__
     __shared__ __align__(16) unsigned char lds[2 * SLOT];
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int c0 = (blockIdx.y * 8 + w) * 16 * CT;
+    const int c0 = (cgroup * 8 + w) * 16 * CT;
     const bool computes = c0 < ldb;                /* wave-uniform */
     const int kofs = (lane >> 4) * 8;
     i16x8 bhi[CT][KS], blo[CT][KS];
@@ -281,9 +284,6 @@ __global__ __launch_bounds__(512) void unmix_lds_kernel(const void* __restrict__
             blo[c][k] = computes ? *(const i16x8*)(Wlo + wofs) : (i16x8){0, 0, 0, 0, 0, 0, 0, 0};
         }
     }
-    const long first = blockIdx.x, stride = gridDim.x;
-    const long ntl = first < ntiles ? (ntiles - 1 - first) / stride + 1 : 0;   /* block-uniform */
-
     auto gload = [&](long j, uint4 (&r)[NCH]) {
         const long tile = first + j * stride;
 #pragma unroll
@@ -370,6 +370,31 @@ __global__ __launch_bounds__(512) void unmix_lds_kernel(const void* __restrict__
             }
         }
     }
+}
+
+template <int CT, int KS, bool Q8>
+__global__ __launch_bounds__(512) void unmix_lds_kernel(const void* __restrict__ xa,
+                                                         const uint16_t* __restrict__ xlo,
+                                                         const uint16_t* __restrict__ Whi,
+                                                         const uint16_t* __restrict__ Wlo,
+                                                         const float* __restrict__ wc,
+                                                         float* __restrict__ b, long T, int ldb,
+                                                         long ntiles) {
+    const long first = blockIdx.x, stride = gridDim.x;
+    const long ntl = first < ntiles ? (ntiles - 1 - first) / stride + 1 : 0;   /* block-uniform */
+    unmix_lds_body<CT, KS, Q8>(xa, xlo, Whi, Wlo, wc, b, T, ldb, first, stride, ntl, (int)blockIdx.y);
+}
+
+/* Many recordings in one launch (same NP, same column groups): workgroup b
+ * runs work item b, the 16-sample tiles [item.t0, item.t1) of recording
+ * item.rec. */
+template <int CT, int KS, bool Q8>
+__global__ __launch_bounds__(512) void unmix_lds_batch_kernel(const UnmixArgs* __restrict__ args,
+                                                               const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const UnmixArgs u = args[it.rec];
+    unmix_lds_body<CT, KS, Q8>(u.xa, u.xlo, u.Whi, u.Wlo, u.wc, u.b, u.T, u.ldb, it.t0, 1,
+                               it.t1 - it.t0, (int)blockIdx.y);
 }
 
 /* FP64 unmixing with register-resident W (the FP32 path's GEMM1): the
@@ -550,17 +575,17 @@ __global__ __launch_bounds__(512) void gx_f32_slab_kernel(const float* __restric
  * g and x element is read from HBM once per slab; FP32 accumulation inside
  * the MFMA (32 products per rounding) over the chunk, FP64 partial per
  * chunk. */
+/* 32-sample blocks [b0, b1) of one recording into its partial slot `slot` */
 template <int CT>
-__global__ __launch_bounds__(512) void gx_bf16_packed_kernel(const uint16_t* __restrict__ g8,
-                                                             const uint16_t* __restrict__ x8,
-                                                             double* __restrict__ part, long nblk,
-                                                             int NP, int ldb, long cblk) {
+__device__ __forceinline__ void gx_bf16_packed_body(const uint16_t* __restrict__ g8,
+                                                    const uint16_t* __restrict__ x8,
+                                                    double* __restrict__ part, const int NP,
+                                                    const int ldb, const long b0, const long b1,
+                                                    const int slot) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r0 = blockIdx.z * 512 + w * 64;      /* grid.z: groups of 512 g columns */
     if (r0 >= ldb) return;                          /* no barriers below */
     const int k0 = blockIdx.y * 16 * CT;
-    const long b0 = (long)blockIdx.x * cblk;
-    const long b1 = min(nblk, b0 + cblk);
     f32x4 acc[4][CT];
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
@@ -593,7 +618,7 @@ __global__ __launch_bounds__(512) void gx_bf16_packed_kernel(const uint16_t* __r
                                                                       __builtin_bit_cast(bf16x8, Bc[ct]),
                                                                       acc[rt][ct], 0, 0, 0);
     }
-    double* out = part + (size_t)blockIdx.x * ldb * NP;
+    double* out = part + (size_t)slot * ldb * NP;
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
@@ -603,6 +628,27 @@ __global__ __launch_bounds__(512) void gx_bf16_packed_kernel(const uint16_t* __r
                 const int row = r0 + rt * 16 + (lane >> 4) * 4 + r;
                 out[(size_t)row * NP + k0 + ct * 16 + (lane & 15)] = (double)acc[rt][ct][r];
             }
+}
+
+template <int CT>
+__global__ __launch_bounds__(512) void gx_bf16_packed_kernel(const uint16_t* __restrict__ g8,
+                                                             const uint16_t* __restrict__ x8,
+                                                             double* __restrict__ part, long nblk,
+                                                             int NP, int ldb, long cblk) {
+    const long b0 = (long)blockIdx.x * cblk;
+    gx_bf16_packed_body<CT>(g8, x8, part, NP, ldb, b0, min(nblk, b0 + cblk), (int)blockIdx.x);
+}
+
+/* Many recordings in one launch (same NP and ldb): workgroup (b, slab, row
+ * group) runs work item b, the 32-sample blocks [item.t0, item.t1) of
+ * recording item.rec into its partial slot item.slot. */
+template <int CT>
+__global__ __launch_bounds__(512) void gx_bf16_batch_kernel(const GxArgs* __restrict__ args,
+                                                            const BatchItem* __restrict__ items,
+                                                            int NP, int ldb) {
+    const BatchItem it = items[blockIdx.x];
+    const GxArgs g = args[it.rec];
+    gx_bf16_packed_body<CT>(g.g8, g.x8, g.part, NP, ldb, it.t0, it.t1, it.slot);
 }
 
 __global__ void pack8_kernel(const uint16_t* __restrict__ xhi, long T, int NP,
@@ -725,6 +771,25 @@ int launch_unmix_bf16x3(const float* x, const uint16_t* xbf, const uint16_t* Whi
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+int launch_unmix_batch(const UnmixArgs* args, const BatchItem* items, int nitems, int NP, int ldb,
+                       hipStream_t st) {
+    int CT, KS;
+    unmix_shape(NP, CT, KS);
+    if (CT == 0 || ldb % (16 * CT) || nitems < 1) return -1;
+    const dim3 grid(nitems, (ldb + 128 * CT - 1) / (128 * CT));
+    if (CT == 4 && KS == 2)
+        hipLaunchKernelGGL((unmix_lds_batch_kernel<4, 2, false>), grid, dim3(512), 0, st, args, items);
+    else if (CT == 4 && KS == 4)
+        hipLaunchKernelGGL((unmix_lds_batch_kernel<4, 4, false>), grid, dim3(512), 0, st, args, items);
+    else if (CT == 2 && KS == 6)
+        hipLaunchKernelGGL((unmix_lds_batch_kernel<2, 6, false>), grid, dim3(512), 0, st, args, items);
+    else if (CT == 2 && KS == 8)
+        hipLaunchKernelGGL((unmix_lds_batch_kernel<2, 8, false>), grid, dim3(512), 0, st, args, items);
+    else
+        return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_unmix_fp8(const uint8_t* xq, const uint16_t* Whi, const uint16_t* Wlo, const float* wc,
                      float* b, long T, int NP, int ldb, int cus, hipStream_t st) {
     if (NP % 32 || ldb % 64) return -1;
@@ -787,6 +852,18 @@ int launch_gx_bf16(const uint16_t* g8, const uint16_t* x8, double* part, long T,
         hipLaunchKernelGGL(gx_bf16_packed_kernel<8>, grid, block, 0, st, g8, x8, part, nblk, NP, ldb, cblk);
     else
         hipLaunchKernelGGL(gx_bf16_packed_kernel<4>, grid, block, 0, st, g8, x8, part, nblk, NP, ldb, cblk);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_gx_bf16_batch(const GxArgs* args, const BatchItem* items, int nitems, int NP, int ldb,
+                         hipStream_t st) {
+    if (NP % 64 || ldb % 64 || nitems < 1) return -1;
+    const int CT = gx_bf16_ct(NP);
+    const dim3 grid(nitems, NP / (16 * CT), (ldb + 511) / 512), block(std::min(ldb, 512));
+    if (CT == 8)
+        hipLaunchKernelGGL(gx_bf16_batch_kernel<8>, grid, block, 0, st, args, items, NP, ldb);
+    else
+        hipLaunchKernelGGL(gx_bf16_batch_kernel<4>, grid, block, 0, st, args, items, NP, ldb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -75,6 +75,7 @@ _SIGNATURES = [
     ("amica_last_error", ctypes.c_char_p, []),
     ("amica_engine_debug_buffer", ctypes.c_int, [_V, ctypes.c_char_p, _V, ctypes.c_size_t]),
     ("amica_engine_cu_count", ctypes.c_int, [_V]),
+    ("amica_engine_set_cu_budget", ctypes.c_int, [_V, ctypes.c_int]),
     ("amica_hbm_probe", ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _D, _D]),
     ("amica_synth_data", ctypes.c_int, [ctypes.POINTER(SynthSpec), _F, _D, _D, ctypes.c_int]),
     ("amica_synth_init", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
@@ -299,6 +300,10 @@ class Engine:
     def cu_count(self):
         return self.L.amica_engine_cu_count(self.h)
 
+    def set_cu_budget(self, cus):
+        """Grids sized for `cus` compute units (0: the whole device)."""
+        _check(self.L.amica_engine_set_cu_budget(self.h, int(cus)), "set_cu_budget")
+
 
 def hbm_probe(device=0, mib=2048, reps=5):
     """Achievable HBM bandwidth (GB/s) of this library's streaming copy
@@ -347,7 +352,9 @@ def cond(A):
 
 BATCH_POLICIES = tuple(f"{mode}, {bud}" for mode in ("one graph", "graph per stream")
                        for bud in ("whole GPU each", "work-proportional CUs", "twice proportional",
-                                   "half GPU each"))
+                                   "half GPU each")) + tuple(
+    f"grouped launches, {k}" for k in ("each recording's own partition", "512 samples per work item",
+                                        "1024 samples per work item", "2048 samples per work item"))
 
 
 class Batch:

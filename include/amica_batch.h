@@ -27,13 +27,20 @@ extern "C" {
 
 typedef struct amica_batch amica_batch;
 
-/* Policies the tuner measures: p = 4 * launch mode + CU budget.
+/* Policies the tuner measures.  p < 8: p = 4 * launch mode + CU budget.
  * Launch modes: 0, the whole batch as one hipGraph (one branch per
  * recording); 1, every recording replays its own one-iteration graph on its
  * own stream (the streams join at the end of amica_batch_iterate).
  * CU budgets: every recording sized for the whole GPU; budgets proportional
- * to each recording's E-step work (M m n T); twice that; half the GPU each. */
-#define AMICA_BATCH_POLICIES 8
+ * to each recording's E-step work (M m n T); twice that; half the GPU each.
+ * p = 8 .. 11: one hipGraph in which the unmixing GEMM, the E-step and the
+ * statistic GEMM of all single-model BF16 recordings of one channel width
+ * run as one launch each over a work list of (recording, sample range) items: each recording's own grid
+ * partition (p = 8, results identical to the recording's own launches), or
+ * items of 512, 1024, 2048 samples (the tile shape the tuner picks); the
+ * per-recording inverse, reductions and M-step are graph branches; other
+ * recordings run as in mode 0. */
+#define AMICA_BATCH_POLICIES 12
 
 /* nrec recordings; opts[r] as for amica_engine_create (same device). */
 int amica_batch_create(const amica_options* opts, int nrec, amica_batch** out);

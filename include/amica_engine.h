@@ -135,6 +135,11 @@ int  amica_engine_debug_buffer(amica_engine* eng, const char* name, void* out, s
 const char* amica_last_error(void);
 /* Number of compute units of the engine's device. */
 int  amica_engine_cu_count(amica_engine* eng);
+/* Sizes the engine's grids for `cus` compute units (<= 0: the whole device;
+ * the E-step then runs (resident workgroups per CU) x cus workgroups, which
+ * the hardware spreads over all CUs).  No counterpart in the reference; the
+ * single-recording form of amica_batch_set_budgets, for occupancy studies. */
+int  amica_engine_set_cu_budget(amica_engine* eng, int cus);
 
 /* Achievable HBM bandwidth on `device`, measured with this library's own
  * streaming kernels over two buffers of `bytes` each (best of `reps`

@@ -94,3 +94,44 @@ def test_batch_tuner_restores_state(built):
             lg = e.log(3)
             assert abs(lg["ll"][0] - first[r]) <= 1e-6 * abs(first[r]), r
             assert np.all(np.isfinite(lg["ll"]))
+
+
+@pytest.mark.parametrize("policy", [8, 10])
+def test_grouped_launches_match_solo_engines(built, policy):
+    """Grouped launches (one unmixing GEMM, one E-step and one statistic GEMM
+    per channel width over a work list of every single-model BF16
+    recording).  With each recording's own partition (policy 8) every
+    partial sum covers the samples it covers in the recording's own
+    launches: all 6 iterations bit for bit.  With 1024-sample items (policy
+    10) the partial sums cover other sample ranges and the FP32 level of the
+    sums rounds differently: the first iteration's LL (the E-step from the
+    same state) within 1e-7, the second within 1e-5.  The 2-model recording
+    runs its own branch in this mode: bit for bit."""
+    iters = 6
+    _say(f"[batch grouped] {len(RECS)} recordings, BF16, {iters} iterations, {ca.BATCH_POLICIES[policy]}")
+    opts, data, states = _setup(ca.PREC_BF16, iters)
+    solo = []
+    for o, x, st in zip(opts, data, states):
+        with ca.Engine(o) as e:
+            _load(e, x, st)
+            e.iterate(iters)
+            e.sync()
+            solo.append((e.log(iters), e.get_state()))
+    with ca.Batch(opts) as b:
+        for e, x, st in zip(b.engines, data, states):
+            _load(e, x, st)
+        b.set_policy(policy)
+        b.iterate(iters)
+        b.sync()
+        for r, e in enumerate(b.engines):
+            lg, st = e.log(iters), e.get_state()
+            rel = np.abs(lg["ll"] - solo[r][0]["ll"]) / np.abs(solo[r][0]["ll"])
+            _say(f"  recording {r} (n={RECS[r][0]}, T={RECS[r][1]}, M={RECS[r][2]}): LL rel. "
+                 f"difference to the solo engine {np.array2string(rel, precision=2)}")
+            if RECS[r][2] > 1 or policy == 8:
+                assert np.array_equal(lg["ll"], solo[r][0]["ll"]), r
+                for k in ("A", "c", "alpha", "mu", "beta", "rho", "gm"):
+                    assert np.array_equal(st[k], solo[r][1][k]), (r, k)
+            else:
+                assert rel[0] <= 1e-7, r
+                assert rel[1] <= 1e-5, r

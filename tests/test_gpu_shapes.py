@@ -119,15 +119,17 @@ def test_c4_workload_run_follows_oracle(built, precision):
     assert np.all(rel <= np.maximum(1e-6, 4.0 * env))
 
 
-@pytest.mark.parametrize("precision", [ca.PREC_FP32, ca.PREC_BF16], ids=["fp32", "bf16"])
-def test_batch_workload_each_recording_follows_oracle(built, precision):
+@pytest.mark.parametrize("precision,policy", [(ca.PREC_FP32, 0), (ca.PREC_BF16, 0), (ca.PREC_BF16, 8)],
+                         ids=["fp32", "bf16", "bf16-grouped"])
+def test_batch_workload_each_recording_follows_oracle(built, precision, policy):
     """The batched-recordings workload's generator (n drawn from {32, 64,
     128}, M = 1, m = 3, each recording its own seed), shortened to
     T in [20k, 50k]: six recordings iterated as one batch, each one's LL
     trajectory checked against its own FP64 oracle run: the first iteration
     within 1e-7 (FP32) / 1e-6 (BF16), later ones within max(1e-6, 4x the
     oracle's spread under float32-sized (FP32) or 2^-17 (BF16) input
-    perturbations)."""
+    perturbations); BF16 also with the grouped launches over all recordings
+    (each recording's own partition)."""
     iters = 4
     recs = ca.batch_workload(count=6, precision=precision, t_range=(20_000, 50_000), iters=iters)
     data, states, opts = [], [], []
@@ -142,6 +144,7 @@ def test_batch_workload_each_recording_follows_oracle(built, precision):
             e.load_raw(x)
             e.preprocess(1)
             e.set_state(st)
+        b.set_policy(policy)
         b.iterate(iters)
         b.sync()
         logs = [e.log(iters) for e in b.engines]
