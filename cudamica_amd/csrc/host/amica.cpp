@@ -126,40 +126,57 @@ extern "C" error runamica(void) {
     }
 
     std::vector<double> ll(maxiter, 0.0);
-    std::vector<double> vsum(M), lr(1);
-    std::vector<int> bflag(M), newt(M), dec(1);
-    std::vector<double> ll_all(maxiter);
+    std::vector<double> vs(M);
+    std::vector<int> bfs(M), nws(M);
+    FILE* jsonlog = nullptr;
+    if (currentConfig.jsonlogfile && !(jsonlog = fopen(currentConfig.jsonlogfile, "w"))) {
+        fprintf(stderr, "ERROR: cannot open %s\n", currentConfig.jsonlogfile);
+        return ERRORNOFILE;
+    }
     fprintf(stdout, "Starting %lu AMICA iterations\n", maxiter);
     for (natural iter = 1; iter <= maxiter; ++iter) {
         const clock_t c0 = clock();
         const auto t0 = std::chrono::steady_clock::now();
-        if (iter == maxiter && currentConfig.ltallfile) amica_engine_record_ltall(eng, 1);
-        if (amica_engine_iterate(eng, 1) != 0 || amica_engine_sync(eng) != 0) {
+        /* one captured hipGraph per iteration; the last iteration records the
+         * per-sample L_h when ltall is written (not part of the graph) */
+        const bool last_ltall = iter == maxiter && currentConfig.ltallfile;
+        if (last_ltall) amica_engine_record_ltall(eng, 1);
+        const int irc = last_ltall ? amica_engine_iterate(eng, 1) : amica_engine_iterate_graph(eng, 1);
+        if (irc != 0 || amica_engine_sync(eng) != 0) {
             fprintf(stderr, "ERROR: iteration %lu: %s\n", iter, amica_last_error());
             exit(1);
         }
-        std::vector<double> lls(iter), lrs(iter), vs(iter * M);
-        std::vector<int> bfs(iter * M), nws(iter * M), decs(iter);
-        if (amica_engine_get_log(eng, (int)iter, lls.data(), lrs.data(), bfs.data(), nws.data(),
-                                 decs.data(), vs.data()) != 0) {
+        /* this iteration's log entry only */
+        double lli = 0.0, lri = 0.0;
+        int deci = 0;
+        if (amica_engine_get_log_at(eng, (int)iter, &lli, &lri, bfs.data(), nws.data(), &deci, vs.data()) != 0) {
             fprintf(stderr, "ERROR: %s\n", amica_last_error());
             exit(1);
         }
-        const size_t k = iter - 1;
-        ll[k] = lls[k];
+        ll[iter - 1] = lli;
         for (natural h = 0; h < M; ++h)
-            fprintf(stdout, "mmodel %lu -> vsum %.16f\n", h, vs[k * M + h]);
-        fprintf(stdout, "Iteration %lu: lrate %.16f LL %.16f \n", iter, lrs[k], lls[k]);
-        if (decs[k]) fprintf(stdout, "Likelihood decreasing!\n");
+            fprintf(stdout, "mmodel %lu -> vsum %.16f\n", h, vs[h]);
+        fprintf(stdout, "Iteration %lu: lrate %.16f LL %.16f \n", iter, lri, lli);
+        if (deci) fprintf(stdout, "Likelihood decreasing!\n");
         for (natural h = 0; h < M; ++h)
-            if (bfs[k * M + h] != 0 && iter >= currentConfig.newt_start_iter)
+            if (bfs[h] != 0 && iter >= currentConfig.newt_start_iter)
                 fprintf(stdout, "Hessian non positive definite! Using natural gradient!\n");
         const auto t1 = std::chrono::steady_clock::now();
-        const long ms = (long)std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();
+        const double msd = std::chrono::duration<double, std::milli>(t1 - t0).count();
         fprintf(stdout, "Iteration %lu times = %lu ticks, %lu ms \n", iter,
-                (unsigned long)(clock() - c0), (unsigned long)ms);
+                (unsigned long)(clock() - c0), (unsigned long)msd);
         fflush(stdout);
+        if (jsonlog) {
+            fprintf(jsonlog, "{\"iteration\": %lu, \"ll\": %.17g, \"lrate\": %.17g, \"ll_decrease\": %d, "
+                             "\"ms\": %.4f, \"models\": [", iter, lli, lri, deci, msd);
+            for (natural h = 0; h < M; ++h)
+                fprintf(jsonlog, "%s{\"vsum\": %.17g, \"bflag\": %d, \"newton\": %d}", h ? ", " : "", vs[h],
+                        bfs[h], nws[h]);
+            fprintf(jsonlog, "]}\n");
+            fflush(jsonlog);
+        }
     }
+    if (jsonlog) fclose(jsonlog);
 
     /* results (amica.cu:2542-2816) */
     if (amica_engine_get_state(eng, st.A.data(), st.c.data(), st.alpha.data(), st.mu.data(),

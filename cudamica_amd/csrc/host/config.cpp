@@ -7,7 +7,8 @@
  * to DEFAULT_UPDATE_RHO and leaves update_rho at 0 (config.c:146-153), so a
  * file without `update_rho` runs with rho fixed.  Keys this engine adds
  * (unknown to the reference parser, which ignores them): `precision`
- * (fp32 | bf16), `seed` (initial-state PRNG), `ainitfile` / `cinitfile`
+ * (fp32 | bf16 | fp8), `seed` (initial-state PRNG), `jsonlogfile` (a JSON
+ * line per iteration), `ainitfile` / `cinitfile`
  * (float64 initial A and c; the reference declares these fields but forces
  * them to NULL, config.c:131-134).
  */
@@ -115,6 +116,7 @@ error getConfig(char* filename) {
         PARSE_FILE(rhofile)
         PARSE_FILE(ainitfile)
         PARSE_FILE(cinitfile)
+        PARSE_FILE(jsonlogfile)
         currentConfig.kinitfile = NULL;
 
         OPT_BOOL("do_newton", do_newton, DEFAULT_DO_NEWTON, "Invalid update policy (doNewton)")
@@ -221,6 +223,7 @@ error freeConfig(void) {
     SAFE_FREE(currentConfig.betafile);
     SAFE_FREE(currentConfig.rhofile);
     SAFE_FREE(currentConfig.ainitfile);
+    SAFE_FREE(currentConfig.jsonlogfile);
     SAFE_FREE(currentConfig.cinitfile);
     SAFE_FREE(currentConfig.kinitfile);
     return SUCCESS;
@@ -232,7 +235,7 @@ void help(void) {
     printf("  optional keys: devices do_sphere maxiter nsdm nmmodels block_size fix_init\n");
     printf("                 do_newton update_rho rho_start_iter newt_start_iter\n");
     printf("                 afile weightsfile cfile llfile ltallfile gmfile alphafile\n");
-    printf("                 mufile betafile rhofile ainitfile cinitfile precision seed\n");
+    printf("                 mufile betafile rhofile ainitfile cinitfile precision seed jsonlogfile\n");
 }
 
 static const char* kErrors[] = {"Success!",     "No device memory", "No parameter",

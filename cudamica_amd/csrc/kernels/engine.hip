@@ -721,6 +721,22 @@ int amica_engine_get_log(amica_engine* e, int n_iter, double* ll, double* lrate,
     return 0;
 }
 
+int amica_engine_get_log_at(amica_engine* e, int it, double* ll, double* lrate, int* bflag,
+                            int* newton, int* lldec, double* vsum) {
+    if (!e) return fail(AMICA_E_INVALID, "null engine");
+    if (it < 1 || it > e->o.max_iter) return fail(AMICA_E_INVALID, "iteration outside the log capacity");
+    HIPCHK(hipSetDevice(e->o.device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    const size_t k = (size_t)it - 1, M = e->o.M;
+    if (ll) HIPCHK(hipMemcpy(ll, e->log_ll + k, sizeof(double), hipMemcpyDeviceToHost));
+    if (lrate) HIPCHK(hipMemcpy(lrate, e->log_lrate + k, sizeof(double), hipMemcpyDeviceToHost));
+    if (bflag) HIPCHK(hipMemcpy(bflag, e->log_bflag + k * M, M * sizeof(int), hipMemcpyDeviceToHost));
+    if (newton) HIPCHK(hipMemcpy(newton, e->log_newton + k * M, M * sizeof(int), hipMemcpyDeviceToHost));
+    if (lldec) HIPCHK(hipMemcpy(lldec, e->log_lldec + k, sizeof(int), hipMemcpyDeviceToHost));
+    if (vsum) HIPCHK(hipMemcpy(vsum, e->log_vsum + k * M, M * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int amica_engine_estep_stats(amica_engine* e, double* stats) {
     int rc = check_ready(e);
     if (rc) return rc;

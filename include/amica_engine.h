@@ -105,6 +105,11 @@ int  amica_engine_iterations(amica_engine* eng);
 int  amica_engine_get_log(amica_engine* eng, int n_iter, double* ll, double* lrate,
                           int* bflag, int* newton, int* lldec, double* vsum);
 
+/* The log entry of iteration `it` (1-based) alone: LL, lrate, lldec and per
+ * model bflag, Newton flag and vsum (M each); any pointer may be NULL. */
+int  amica_engine_get_log_at(amica_engine* eng, int it, double* ll, double* lrate, int* bflag,
+                             int* newton, int* lldec, double* vsum);
+
 /* Runs only the E-step with the current state and returns the canonical
  * statistics (per model: vsum, usum, kappa_numer, lambda_numer, mu_numer,
  * mu_denom, beta_numer, beta_denom, rho_numer (m*n each), sigma2_numer,

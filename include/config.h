@@ -115,6 +115,7 @@ typedef struct config_t {
     /* additions of this engine */
     natural precision;      /* key `precision`: fp32 (32), bf16 (16) or fp8 (8) */
     natural seed;           /* key `seed`: PRNG seed of the initial state */
+    char*   jsonlogfile;    /* key `jsonlogfile`: one JSON line per iteration, or NULL */
 } config;
 
 #ifdef __cplusplus

@@ -2,7 +2,9 @@
 reference's stdout lines and raw float64 result files out.  The files must
 be exactly what the engine's C ABI computes from the same inputs, and the
 LL trajectory must follow the CPU oracle on the same data and initial
-state."""
+state.  The program replays one captured hipGraph per iteration, reads
+only that iteration's log entry, and writes one JSON line per iteration
+(`jsonlogfile`)."""
 import os
 import re
 import subprocess
@@ -30,7 +32,7 @@ def test_cli_end_to_end(built, tmp_path):
     lines = [f"datafile {d}/data.fdt", f"nchannels {n}", f"nsamples {T}", f"nmmodels {M}",
              f"nsdm {m}", f"maxiter {iters}", f"spherefile {d}/out.sph", "do_sphere 1",
              "update_rho 1", "do_newton on", "newt_start_iter 3", f"ainitfile {d}/A0.bin",
-             f"cinitfile {d}/c0.bin", "# a comment line", "unknownkey 5"]
+             f"cinitfile {d}/c0.bin", "# a comment line", "unknownkey 5", f"jsonlogfile {d}/iters.jsonl"]
     lines += [f"{k} {d}/out.{k}" for k in outs]
     with open(os.path.join(d, "run.param"), "w") as f:
         f.write("\n".join(lines) + "\n")
@@ -66,6 +68,15 @@ def test_cli_end_to_end(built, tmp_path):
         return a
 
     np.testing.assert_allclose(rd("llfile", iters), ll_cli, rtol=1e-15)
+    # one JSON line per iteration with the same LL, lrate and per-model vsum
+    import json
+    with open(os.path.join(d, "iters.jsonl")) as f:
+        js = [json.loads(line) for line in f]
+    assert [j["iteration"] for j in js] == list(range(1, iters + 1))
+    np.testing.assert_array_equal([j["ll"] for j in js], elog["ll"])
+    np.testing.assert_array_equal([j["lrate"] for j in js], elog["lrate"])
+    np.testing.assert_array_equal([[mm["vsum"] for mm in j["models"]] for j in js], elog["vsum"])
+    assert all(j["ms"] > 0 for j in js)
     np.testing.assert_allclose(rd("weightsfile", n * n * M), est["W"], rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(rd("afile", n * n * M), Aout, rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(rd("cfile", n * M), cout, rtol=1e-12, atol=1e-14)
