@@ -17,8 +17,10 @@
  * with f = rho sign(y)|y|^(rho-1).
  *
  * Kernels: prep (per model: W^-1 by LU, log|det W|, W c, the constant part
- * of Q), ll (one workgroup of ceil(n/64) waves per model over a range of
- * samples; x tiles in LDS; b = W x - W c by each lane for its channel),
+ * of Q), ll (one wave per (model, 64-channel block) over a range of
+ * samples, all M ceil(n/64) waves in one workgroup where they fit in 8, else
+ * three passes through per-sample buffers; x tiles in LDS; b = W x - W c by
+ * each lane for its channel),
  * gx (sum_t g_t x_t^T, 64 x 64 FP64 tiles over sample chunks), and finish
  * (fixed-order sums over workgroups and the assembly of the gradients).
  * Every reduction is in a fixed order, so results repeat bit for bit.
@@ -58,7 +60,15 @@ struct AdjArgs {
     double* part;        /* [nblocks][pstride] */
     long chunk;          /* samples per workgroup */
     size_t pstride;
+    /* split passes (M ceil(n/64) > 8 waves): workgroups of pg (model,
+     * channel-block) pairs, blockIdx.y = pair group; qbuf [T][M CW] the
+     * pairs' channel sums of q, vbuf [T][M] dll_t v_ht */
+    double* qbuf;
+    double* vbuf;
+    int pg;
 };
+
+enum { kLlFused = 0, kLlQ = 1, kLlGrad = 2 };
 
 /* per pair (h, i): dalpha_j, dmu_j, dbeta_j, drho_j (j < m), gsum; then Omega_h */
 __host__ __device__ inline int pair_stride(int m) { return 4 * m + 1; }
@@ -111,16 +121,32 @@ __device__ __forceinline__ void density(Real b, Real sb, Real mu, Real rho, Real
     Q = cq - yr;
 }
 
-template <typename Real, int MJ, bool ADJ>
+/* MODE kLlFused: a workgroup holds all M ceil(n/64) (model, channel-block)
+ * pairs of a sample range and exchanges the channel and model sums in LDS.
+ * Where they do not fit one workgroup the same body runs in three passes:
+ * kLlQ writes each pair's channel sum of q per sample (qbuf),
+ * ll_post_kernel forms L_h, ll_t and dll_t v_ht (vbuf), and kLlGrad
+ * re-evaluates the densities and accumulates the gradients. */
+template <typename Real, int MJ, bool ADJ, int MODE>
 __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
     constexpr int S = Tile<Real>::S;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = a.n, NP = a.NP, M = a.M;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
     const int CW = NP / 64;
-    const int h = wv / CW;
-    const int i = (wv - h * CW) * 64 + lane;
-    const bool act = i < n;
+    int h, pair = 0;
+    bool wreal = true;   /* wave-uniform: the wave owns a real pair */
+    if constexpr (MODE == kLlFused) {
+        h = wv / CW;
+        pair = wv;
+    } else {
+        pair = blockIdx.y * a.pg + wv;
+        wreal = pair < M * CW;
+        if (!wreal) pair = M * CW - 1;
+        h = pair / CW;
+    }
+    const int i = (pair - h * CW) * 64 + lane;
+    const bool act = wreal && i < n;
     /* LDS: xs[S][NP] (Real), qp[S][nw], Lh[S][M], vh[S][M] (double) */
     Real* xs = (Real*)smem_raw;
     double* qp = (double*)(smem_raw + (size_t)S * NP * sizeof(Real));
@@ -169,8 +195,10 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
                 for (int s = 0; s < S; ++s) bv[s] = fma(wik, xs[s * NP + k], bv[s]);
             }
 #pragma unroll
+        for (int s = 0; s < S; ++s) bv[s] -= wci;
+        if constexpr (MODE != kLlGrad) {
+#pragma unroll
         for (int s = 0; s < S; ++s) {
-            bv[s] -= wci;
             double q = 0.0;
             if (act) {
                 Real Qj[MJ], mx = (Real)0;
@@ -186,8 +214,19 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
                 q = (double)mx + (double)r_log<Real>(se);
             }
             q = wave_sum_f64(q);
-            if (lane == 0) qp[s * nw + wv] = q;
+            if constexpr (MODE == kLlQ) {
+                const long t = tb + s;
+                if (lane == 0 && wreal && t < t1) a.qbuf[(size_t)t * M * CW + pair] = q;
+            } else if constexpr (MODE == kLlFused) {
+                if (lane == 0) qp[s * nw + wv] = q;
+            }
         }
+        }
+        if constexpr (MODE == kLlQ) {
+            __syncthreads();            /* xs is rewritten by the next tile */
+            continue;
+        }
+        if constexpr (MODE == kLlFused) {
         __syncthreads();
         if (tid < S * M) {
             const int s = tid / M, hh = tid - (tid / M) * M;
@@ -210,6 +249,7 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
             for (int hh = 0; hh < M; ++hh) vh[s * M + hh] = w * exp(Lh[s * M + hh] - P);
         }
         __syncthreads();
+        }
         if constexpr (ADJ) {
             Real* g = (Real*)a.g;
             const size_t ldg = (size_t)M * NP;
@@ -217,7 +257,9 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
             for (int s = 0; s < S; ++s) {
                 const long t = tb + s;
                 if (t >= t1) break;
-                const double omh = vh[s * M + h];
+                double omh;
+                if constexpr (MODE == kLlGrad) omh = a.vbuf[(size_t)t * M + h];
+                else omh = vh[s * M + h];
                 double gv = 0.0;
                 if (act) {
                     Real Qj[MJ], Yj[MJ], YRj[MJ], LNj[MJ], mx = (Real)0;
@@ -248,14 +290,15 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
                         gv += sbj * u * f;
                     }
                 }
-                g[(size_t)t * ldg + (size_t)h * NP + i] = (Real)gv;
+                if (wreal) g[(size_t)t * ldg + (size_t)h * NP + i] = (Real)gv;
                 gsum += gv;
                 if (i == 0) om += omh;
             }
         }
         __syncthreads();
     }
-    if constexpr (ADJ) {
+    if constexpr (ADJ && MODE != kLlQ) {
+        if (!wreal) return;
         double* base = a.part + (size_t)blockIdx.x * a.pstride;
         double* pp = base + ((size_t)h * NP + i) * pair_stride(MJ);
 #pragma unroll
@@ -267,6 +310,38 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
         }
         pp[4 * MJ] = gsum;
         if (i == 0) base[(size_t)M * NP * pair_stride(MJ) + h] = om;
+    }
+}
+
+/* The split passes' middle step, one thread per sample: L_h = log gm_h +
+ * log|det W_h| + ldetS + the pairs' sums of q, ll_t = logsumexp_h L_h and
+ * (adjoint) dll_t v_ht. */
+__global__ __launch_bounds__(256) void ll_post_kernel(AdjArgs a) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.T) return;
+    const int M = a.M, CW = a.NP / 64;
+    const double* q = a.qbuf + (size_t)t * M * CW;
+    double mx = -HUGE_VAL;
+    for (int hh = 0; hh < M; ++hh) {
+        double L = log(a.gm[hh]) + a.ldet[hh] + a.ldetS;
+        for (int c = 0; c < CW; ++c) L += q[hh * CW + c];
+        mx = fmax(mx, L);
+    }
+    double se = 0.0;
+    for (int hh = 0; hh < M; ++hh) {
+        double L = log(a.gm[hh]) + a.ldet[hh] + a.ldetS;
+        for (int c = 0; c < CW; ++c) L += q[hh * CW + c];
+        se += exp(L - mx);
+    }
+    const double P = mx + log(se);
+    if (a.ll) a.ll[t] = P;
+    if (a.dll) {
+        const double w = a.dll[t];
+        for (int hh = 0; hh < M; ++hh) {
+            double L = log(a.gm[hh]) + a.ldet[hh] + a.ldetS;
+            for (int c = 0; c < CW; ++c) L += q[hh * CW + c];
+            a.vbuf[(size_t)t * M + hh] = w * exp(L - P);
+        }
     }
 }
 
@@ -387,15 +462,14 @@ __global__ __launch_bounds__(256) void adj_finish_kernel(AdjArgs a, int nblocks,
     }
 }
 
-template <typename Real, bool ADJ>
-int launch_ll(int MJ, const AdjArgs& a, int nblocks, size_t shm, hipStream_t st) {
-    dim3 grid(nblocks), block(64 * a.M * (a.NP / 64));
+template <typename Real, bool ADJ, int MODE>
+int launch_ll_mode(int MJ, const AdjArgs& a, dim3 grid, dim3 block, size_t shm, hipStream_t st) {
 #define AMICA_LL_CASE(J)                                                                           \
     case J:                                                                                        \
-        if (hipFuncSetAttribute((const void*)ll_kernel<Real, J, ADJ>,                              \
+        if (hipFuncSetAttribute((const void*)ll_kernel<Real, J, ADJ, MODE>,                        \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) != hipSuccess) \
             return -1;                                                                             \
-        hipLaunchKernelGGL((ll_kernel<Real, J, ADJ>), grid, block, shm, st, a);                   \
+        hipLaunchKernelGGL((ll_kernel<Real, J, ADJ, MODE>), grid, block, shm, st, a);             \
         break;
     switch (MJ) {
         AMICA_LL_CASE(1)
@@ -412,14 +486,29 @@ int launch_ll(int MJ, const AdjArgs& a, int nblocks, size_t shm, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <typename Real, bool ADJ>
+int launch_ll(int MJ, const AdjArgs& a, int nblocks, size_t shm, hipStream_t st) {
+    const int pairs = a.M * (a.NP / 64);
+    if (!a.qbuf)
+        return launch_ll_mode<Real, ADJ, kLlFused>(MJ, a, dim3(nblocks), dim3(64 * pairs), shm, st);
+    const dim3 grid(nblocks, (pairs + a.pg - 1) / a.pg), block(64 * a.pg);
+    const size_t shx = (size_t)Tile<Real>::S * a.NP * sizeof(Real);
+    if (launch_ll_mode<Real, false, kLlQ>(MJ, a, grid, block, shx, st)) return -1;
+    hipLaunchKernelGGL(ll_post_kernel, dim3((unsigned)((a.T + 255) / 256)), dim3(256), 0, st, a);
+    if (hipGetLastError() != hipSuccess) return -1;
+    return ADJ ? launch_ll_mode<Real, true, kLlGrad>(MJ, a, grid, block, shx, st) : 0;
+}
+
 /* Workspace of one call, freed on every exit path. */
 struct Workspace {
-    void* ptrs[8] = {nullptr};
+    static constexpr int kMax = 12;
+    void* ptrs[kMax] = {nullptr};
     int n = 0;
     ~Workspace() {
         for (int k = 0; k < n; ++k) (void)hipFree(ptrs[k]);
     }
     bool alloc(void** p, size_t bytes) {
+        if (n == kMax) return false;
         if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
         ptrs[n++] = *p;
         return true;
@@ -478,8 +567,12 @@ int ll_call(int real_bits, const void* x, long T, int n, int M, int m, const dou
     if (!adj && !ll) return report_error(AMICA_E_INVALID, "null ll output");
     if (T < 1 || n < 1 || M < 1 || m < 1 || m > 8)
         return report_error(AMICA_E_INVALID, "need T >= 1, n >= 1, M >= 1, 1 <= m <= 8");
+    if (n > 1024 || M > 16) return report_error(AMICA_E_INVALID, "need n <= 1024 and M <= 16");
     const int NP = (n + 63) / 64 * 64;
-    if ((long)M * NP > 512) return report_error(AMICA_E_INVALID, "M * ceil(n/64) * 64 must be <= 512");
+    /* all (model, channel-block) pairs in one workgroup, else the split passes */
+    const bool split = (long)M * NP > 512;
+    if (split && noalloc)
+        return report_error(AMICA_E_INVALID, "M * ceil(n/64) * 64 must be <= 512 without allocation");
     hipStream_t st = (hipStream_t)stream;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -518,6 +611,12 @@ int ll_call(int real_bits, const void* x, long T, int n, int M, int m, const dou
                !ws.alloc((void**)&a.cq, (size_t)M * m * n * 8) || !ws.alloc((void**)&a.wc, (size_t)M * NP * 8) ||
                !ws.alloc((void**)&a.lu, M * nn * 8) || !ws.alloc((void**)&a.X, M * nn * 8)) {
         return report_error(AMICA_E_NOMEM, "workspace allocation failed");
+    }
+    if (split) {
+        a.pg = 8;
+        if (!ws.alloc((void**)&a.qbuf, (size_t)T * (M * NP / 64) * 8) ||
+            (adj && !ws.alloc((void**)&a.vbuf, (size_t)T * M * 8)))
+            return report_error(AMICA_E_NOMEM, "workspace allocation failed");
     }
     const int gchunks = (int)std::max(1L, std::min((long)cus, (T + 255) / 256));
     double* gxpart = nullptr;

@@ -37,7 +37,9 @@
  *   dll        [T] FP64 upstream gradient
  *   dW ... dgm FP64 outputs, the shapes of their parameters
  *   stream     hipStream_t the work is enqueued on (NULL: the default stream)
- * Limits: 1 <= m <= 8, M * ceil(n/64) * 64 <= 512.  The call allocates its
+ * Limits: 1 <= m <= 8, n <= 1024, M <= 16 (the reference's range).  Where
+ * M * ceil(n/64) * 64 > 512 the likelihood runs in three passes through
+ * per-sample buffers of (M ceil(n/64) + M) doubles.  The call allocates its
  * workspace, synchronises `stream` before it returns and returns 0, or a
  * negative AMICA_E_* code of amica_engine.h with amica_last_error() set.
  */

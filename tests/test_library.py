@@ -100,7 +100,9 @@ def test_ll_entry_points_validate_before_touching_gpu(built):
     p = [1] * 7
     assert L.amica_ll_forward(16, 1, 10, 4, 1, 3, *p, 0.0, 1, None) == -3        # real_bits
     assert L.amica_ll_forward(32, 1, 10, 4, 1, 9, *p, 0.0, 1, None) == -3        # m > 8
-    assert L.amica_ll_forward(32, 1, 10, 300, 2, 3, *p, 0.0, 1, None) == -3      # M NP > 512
+    assert L.amica_ll_forward(32, 1, 10, 1025, 1, 3, *p, 0.0, 1, None) == -3     # n > 1024
+    assert L.amica_ll_forward(32, 1, 10, 16, 17, 3, *p, 0.0, 1, None) == -3      # M > 16
+    assert L.amica_ll_forward_noalloc(32, 1, 10, 300, 2, 3, *p, 0.0, 1, None) == -3  # M NP > 512
     assert L.amica_ll_forward(64, 1, 10, 4, 1, 3, *p, 0.0, None, None) == -3     # no ll output
     assert b"ll output" in L.amica_last_error()
     assert L.amica_ll_adjoint(64, 1, 10, 4, 1, 3, *p, 0.0, None, None, *p, None) == -3  # no dll
