@@ -1,7 +1,8 @@
 /*
  * config.cpp -- run configuration from the parameter file, with the keys,
  * defaults, mandatory keys and messages of fraimondo/cudamica
- * src/config.c:12-267.
+ * src/config.c:12-267.  One table describes every key (how it is read,
+ * what its absence means); getConfig walks it.
  *
  * Kept as the reference has it: a missing `update_rho` key sets do_newton
  * to DEFAULT_UPDATE_RHO and leaves update_rho at 0 (config.c:146-153), so a
@@ -14,151 +15,152 @@
  */
 #include "../../../include/config.h"
 
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
 #include "../../../include/params.h"
 
-extern "C" {
+namespace {
 
-config currentConfig;
-char* programName;
+/* How a key is read and what its absence means. */
+enum KeyKind {
+    kRequiredString,   /* absent: error */
+    kRequiredNatural,  /* absent: error */
+    kNatural,          /* absent: the default */
+    kBool,             /* absent: the default */
+    kFile,             /* absent: NULL */
+    kRhoPolicy         /* update_rho, kept as the reference has it: its absence
+                        * sets do_newton to the default and leaves update_rho at
+                        * 0 (config.c:146-153) */
+};
 
-error getConfig(char* filename) {
-    error retorno = SUCCESS;
-    error newerr = SUCCESS;
-    parsedconfigs* pc = NULL;
-    std::memset(&currentConfig, 0, sizeof(currentConfig));
+struct KeySpec {
+    const char* key;
+    KeyKind kind;
+    size_t offset;      /* of the config field */
+    natural fallback;   /* kNatural / kBool */
+    const char* message;
+};
 
-    retorno = parseConfig(filename, &pc);
-    if (retorno == SUCCESS) {
-        natural devs[64];
-        natural ndev = 0;
-        if ((newerr = getNaturalList(pc, (char*)"devices", devs, &ndev)) != SUCCESS) {
-            if (newerr == ERRORNOPARAM) {
-                currentConfig.devcount = 1;
-                currentConfig.devs[0] = 0;
-            } else {
-                fprintf(stderr, "ERROR: Invalid devices specification (devices)\n");
-                retorno = newerr;
-            }
-        } else {
-            if (ndev > MAX_DEVS) ndev = MAX_DEVS;
-            currentConfig.devcount = ndev;
-            for (natural i = 0; i < ndev; ++i) currentConfig.devs[i] = devs[i];
-        }
+#define AMICA_FIELD(f) offsetof(config, f)
 
-        if ((newerr = getString(pc, (char*)"datafile", &currentConfig.datafile)) != SUCCESS) {
-            fprintf(stderr, "ERROR: Invalid data file (datafile)\n");
-            retorno = newerr;
-        }
-        if ((newerr = getNatural(pc, (char*)"nchannels", &currentConfig.nchannels)) != SUCCESS) {
-            fprintf(stderr, "ERROR: Invalid number of channels (nchannels)\n");
-            retorno = newerr;
-        }
-        if ((newerr = getNatural(pc, (char*)"nsamples", &currentConfig.nsamples)) != SUCCESS) {
-            fprintf(stderr, "ERROR: Invalid number of samples (nsamples)\n");
-            retorno = newerr;
-        }
+/* Keys in the order their errors are reported (config.c:27-176); the
+ * messages are the reference's, so scripts that match them keep working. */
+const KeySpec kKeys[] = {
+    {"datafile", kRequiredString, AMICA_FIELD(datafile), 0, "Invalid data file (datafile)"},
+    {"nchannels", kRequiredNatural, AMICA_FIELD(nchannels), 0, "Invalid number of channels (nchannels)"},
+    {"nsamples", kRequiredNatural, AMICA_FIELD(nsamples), 0, "Invalid number of samples (nsamples)"},
+    {"do_sphere", kNatural, AMICA_FIELD(do_sphere), DEFAULT_DO_SPHERE, "Invalid sphering value (do_sphere)"},
+    {"maxiter", kNatural, AMICA_FIELD(maxiter), DEFAULT_MAXITER,
+     "Invalid number of maximum iterations (maxiter)"},
+    {"nsdm", kNatural, AMICA_FIELD(nsdm), DEFAULT_NSDM, "Invalid number of source density modles (nsdm)"},
+    {"nmmodels", kNatural, AMICA_FIELD(nmmodels), DEFAULT_NMMODELS,
+     "Invalid number of ICA mixture models (nmmodels)"},
+    {"spherefile", kRequiredString, AMICA_FIELD(spherefile), 0, "Invalid sphere file (spherefile)"},
+    {"block_size", kNatural, AMICA_FIELD(block_size), DEFAULT_BLOCK_SIZE, "Invalid block size (block_size)"},
+    {"fix_init", kBool, AMICA_FIELD(fix_init), DEFAULT_FIX_INIT, "Invalid fix_init value (fix_init)"},
+    {"afile", kFile, AMICA_FIELD(afile), 0, "Invalid file (afile)"},
+    {"weightsfile", kFile, AMICA_FIELD(weightsfile), 0, "Invalid file (weightsfile)"},
+    {"khindsfile", kFile, AMICA_FIELD(khindsfile), 0, "Invalid file (khindsfile)"},
+    {"cfile", kFile, AMICA_FIELD(cfile), 0, "Invalid file (cfile)"},
+    {"llfile", kFile, AMICA_FIELD(llfile), 0, "Invalid file (llfile)"},
+    {"ltallfile", kFile, AMICA_FIELD(ltallfile), 0, "Invalid file (ltallfile)"},
+    {"gmfile", kFile, AMICA_FIELD(gmfile), 0, "Invalid file (gmfile)"},
+    {"alphafile", kFile, AMICA_FIELD(alphafile), 0, "Invalid file (alphafile)"},
+    {"mufile", kFile, AMICA_FIELD(mufile), 0, "Invalid file (mufile)"},
+    {"betafile", kFile, AMICA_FIELD(betafile), 0, "Invalid file (betafile)"},
+    {"rhofile", kFile, AMICA_FIELD(rhofile), 0, "Invalid file (rhofile)"},
+    {"ainitfile", kFile, AMICA_FIELD(ainitfile), 0, "Invalid file (ainitfile)"},
+    {"cinitfile", kFile, AMICA_FIELD(cinitfile), 0, "Invalid file (cinitfile)"},
+    {"jsonlogfile", kFile, AMICA_FIELD(jsonlogfile), 0, "Invalid file (jsonlogfile)"},
+    {"do_newton", kBool, AMICA_FIELD(do_newton), DEFAULT_DO_NEWTON, "Invalid update policy (doNewton)"},
+    {"update_rho", kRhoPolicy, AMICA_FIELD(update_rho), DEFAULT_UPDATE_RHO,
+     "Invalid rho update policy (update_rho)"},
+    {"rho_start_iter", kNatural, AMICA_FIELD(rho_start_iter), DEFAULT_RHO_START_ITER,
+     "Invalid number of iterations to update rho (rho_start_iter)"},
+    {"newt_start_iter", kNatural, AMICA_FIELD(newt_start_iter), DEFAULT_NEWT_START_ITER,
+     "Invalid number of iterations to do newton (newt_start_iter)"},
+    {"seed", kNatural, AMICA_FIELD(seed), DEFAULT_SEED, "Invalid seed (seed)"},
+};
 
-#define OPT_NATURAL(key, field, def, msg)                                            \
-        if ((newerr = getNatural(pc, (char*)key, &currentConfig.field)) != SUCCESS) { \
-            if (newerr == ERRORNOPARAM) {                                             \
-                currentConfig.field = def;                                            \
-            } else {                                                                  \
-                fprintf(stderr, "ERROR: %s\n", msg);                                  \
-                retorno = newerr;                                                     \
-            }                                                                         \
-        }
-#define OPT_BOOL(key, field, def, msg)                                               \
-        if ((newerr = getBool(pc, (char*)key, &currentConfig.field)) != SUCCESS) {    \
-            if (newerr == ERRORNOPARAM) {                                             \
-                currentConfig.field = def;                                            \
-            } else {                                                                  \
-                fprintf(stderr, "ERROR: %s\n", msg);                                  \
-                retorno = newerr;                                                     \
-            }                                                                         \
-        }
+#undef AMICA_FIELD
 
-        OPT_NATURAL("do_sphere", do_sphere, DEFAULT_DO_SPHERE, "Invalid sphering value (do_sphere)")
-        OPT_NATURAL("maxiter", maxiter, DEFAULT_MAXITER, "Invalid number of maximum iterations (maxiter)")
-        OPT_NATURAL("nsdm", nsdm, DEFAULT_NSDM, "Invalid number of source density modles (nsdm)")
-        OPT_NATURAL("nmmodels", nmmodels, DEFAULT_NMMODELS, "Invalid number of ICA mixture models (nmmodels)")
-
-        if ((newerr = getString(pc, (char*)"spherefile", &currentConfig.spherefile)) != SUCCESS) {
-            fprintf(stderr, "ERROR: Invalid sphere file (spherefile)\n");
-            retorno = newerr;
-        }
-
-        OPT_NATURAL("block_size", block_size, DEFAULT_BLOCK_SIZE, "Invalid block size (block_size)")
-        OPT_BOOL("fix_init", fix_init, DEFAULT_FIX_INIT, "Invalid fix_init value (fix_init)")
-
-#define PARSE_FILE(file)                                                             \
-        if ((newerr = getString(pc, (char*)#file, &currentConfig.file)) != SUCCESS) {  \
-            if (newerr == ERRORNOPARAM) {                                             \
-                currentConfig.file = NULL;                                            \
-            } else {                                                                  \
-                fprintf(stderr, "ERROR: Invalid file (" #file ")\n");                 \
-                retorno = newerr;                                                     \
-            }                                                                         \
-        }
-        PARSE_FILE(afile)
-        PARSE_FILE(weightsfile)
-        PARSE_FILE(khindsfile)
-        PARSE_FILE(cfile)
-        PARSE_FILE(llfile)
-        PARSE_FILE(ltallfile)
-        PARSE_FILE(gmfile)
-        PARSE_FILE(alphafile)
-        PARSE_FILE(mufile)
-        PARSE_FILE(betafile)
-        PARSE_FILE(rhofile)
-        PARSE_FILE(ainitfile)
-        PARSE_FILE(cinitfile)
-        PARSE_FILE(jsonlogfile)
-        currentConfig.kinitfile = NULL;
-
-        OPT_BOOL("do_newton", do_newton, DEFAULT_DO_NEWTON, "Invalid update policy (doNewton)")
-
-        if ((newerr = getNatural(pc, (char*)"update_rho", &currentConfig.update_rho)) != SUCCESS) {
-            if (newerr == ERRORNOPARAM) {
-                currentConfig.do_newton = DEFAULT_UPDATE_RHO;   /* as config.c:149 */
-            } else {
-                fprintf(stderr, "ERROR: Invalid rho update policy (update_rho)\n");
-                retorno = newerr;
-            }
-        }
-
-        OPT_NATURAL("rho_start_iter", rho_start_iter, DEFAULT_RHO_START_ITER,
-                    "Invalid number of iterations to update rho (rho_start_iter)")
-        OPT_NATURAL("newt_start_iter", newt_start_iter, DEFAULT_NEWT_START_ITER,
-                    "Invalid number of iterations to do newton (newt_start_iter)")
-        OPT_NATURAL("seed", seed, DEFAULT_SEED, "Invalid seed (seed)")
-
-        char* prec = NULL;
-        if ((newerr = getString(pc, (char*)"precision", &prec)) == SUCCESS) {
-            if (!strcmp(prec, "fp32") || !strcmp(prec, "32")) currentConfig.precision = 32;
-            else if (!strcmp(prec, "bf16") || !strcmp(prec, "16")) currentConfig.precision = 16;
-            else if (!strcmp(prec, "fp8") || !strcmp(prec, "8")) currentConfig.precision = 8;
-            else {
-                fprintf(stderr, "ERROR: Invalid precision (precision): %s\n", prec);
-                retorno = ERRORINVALIDPARAM;
-            }
-            free(prec);
-        } else if (newerr == ERRORNOPARAM) {
-            currentConfig.precision = DEFAULT_PRECISION;
-        } else {
-            fprintf(stderr, "ERROR: Invalid precision (precision)\n");
-            retorno = newerr;
-        }
-#undef OPT_NATURAL
-#undef OPT_BOOL
-#undef PARSE_FILE
-        freeParsedConfig(pc);
+/* Reads one key into currentConfig; returns SUCCESS or the parser's error
+ * (after printing the key's message). */
+error read_key(parsedconfigs* pc, const KeySpec& k) {
+    char* field = (char*)&currentConfig + k.offset;
+    error rc;
+    switch (k.kind) {
+        case kRequiredString:
+        case kFile: rc = getString(pc, (char*)k.key, (char**)field); break;
+        case kBool: rc = getBool(pc, (char*)k.key, (natural*)field); break;
+        default: rc = getNatural(pc, (char*)k.key, (natural*)field); break;
     }
+    if (rc == SUCCESS) return SUCCESS;
+    const bool optional = k.kind == kNatural || k.kind == kBool || k.kind == kFile;
+    if (rc == ERRORNOPARAM && optional) {
+        if (k.kind == kFile) *(char**)field = NULL;
+        else *(natural*)field = k.fallback;
+        return SUCCESS;
+    }
+    if (rc == ERRORNOPARAM && k.kind == kRhoPolicy) {
+        currentConfig.do_newton = k.fallback;
+        return SUCCESS;
+    }
+    fprintf(stderr, "ERROR: %s\n", k.message);
+    return rc;
+}
 
-    /* fixed defaults (config.c:181-210) */
+/* `devices`: a list of ordinals, default device 0. */
+error read_devices(parsedconfigs* pc) {
+    natural list[64];
+    natural count = 0;
+    const error rc = getNaturalList(pc, (char*)"devices", list, &count);
+    if (rc == ERRORNOPARAM) {
+        currentConfig.devcount = 1;
+        currentConfig.devs[0] = 0;
+        return SUCCESS;
+    }
+    if (rc != SUCCESS) {
+        fprintf(stderr, "ERROR: Invalid devices specification (devices)\n");
+        return rc;
+    }
+    if (count > MAX_DEVS) count = MAX_DEVS;
+    currentConfig.devcount = count;
+    for (natural i = 0; i < count; ++i) currentConfig.devs[i] = list[i];
+    return SUCCESS;
+}
+
+/* `precision`: fp32 | bf16 | fp8 (or 32 | 16 | 8). */
+error read_precision(parsedconfigs* pc) {
+    char* text = NULL;
+    const error rc = getString(pc, (char*)"precision", &text);
+    if (rc == ERRORNOPARAM) {
+        currentConfig.precision = DEFAULT_PRECISION;
+        return SUCCESS;
+    }
+    if (rc != SUCCESS) {
+        fprintf(stderr, "ERROR: Invalid precision (precision)\n");
+        return rc;
+    }
+    static const struct { const char* a; const char* b; natural bits; } kNames[] = {
+        {"fp32", "32", 32}, {"bf16", "16", 16}, {"fp8", "8", 8}};
+    error out = ERRORINVALIDPARAM;
+    for (const auto& e : kNames)
+        if (!strcmp(text, e.a) || !strcmp(text, e.b)) {
+            currentConfig.precision = e.bits;
+            out = SUCCESS;
+        }
+    if (out != SUCCESS) fprintf(stderr, "ERROR: Invalid precision (precision): %s\n", text);
+    free(text);
+    return out;
+}
+
+/* The settings the parameter file cannot change (config.c:181-210). */
+void set_fixed_defaults() {
+    currentConfig.kinitfile = NULL;
     currentConfig.comp_thresh = DEFAULT_COMP_THRESH;
     currentConfig.ident_intvl = DEFAULT_IDENT_INTVL;
     currentConfig.update_gm = DEFAULT_UPDATE_GM;
@@ -179,7 +181,32 @@ error getConfig(char* filename) {
     currentConfig.numdec = DEFAULT_NUMDEC;
     currentConfig.maxdec = DEFAULT_MAXDEC;
     currentConfig.blocks_per_gpu_per_model = (natural)-1;
-    return retorno;
+}
+
+}  // namespace
+
+extern "C" {
+
+config currentConfig;
+char* programName;
+
+/* Every key is read and every bad one reported; the result is the last
+ * error (SUCCESS when all keys were good). */
+error getConfig(char* filename) {
+    std::memset(&currentConfig, 0, sizeof(currentConfig));
+    parsedconfigs* pc = NULL;
+    error result = parseConfig(filename, &pc);
+    if (result == SUCCESS) {
+        auto note = [&result](error rc) {
+            if (rc != SUCCESS) result = rc;
+        };
+        note(read_devices(pc));
+        for (const KeySpec& k : kKeys) note(read_key(pc, k));
+        note(read_precision(pc));
+        freeParsedConfig(pc);
+    }
+    set_fixed_defaults();
+    return result;
 }
 
 void printConfig(void) {
