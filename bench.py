@@ -13,6 +13,12 @@ Prints one JSON line.  Fields beyond the driver's contract:
                 around it on the engine stream during the timed steps; also
                 against the bandwidth of this library's own streaming copy
                 kernel (amica_hbm_probe)
+  valu          the E-step's VALU-issue bound (the one it runs against): SIMD
+                cycles its VALU instructions occupy per launch (rocprofv3
+                SQ_ACTIVE_INST_VALU x 4, from profiles/valu_<workload>_<prec>.json)
+                over the SIMD cycles of 1024 SIMDs at the 2400 MHz maximum engine
+                clock during the live average launch time; frac_under_counters
+                uses the cycles the counter run itself counted (GRBM_GUI_ACTIVE)
   gemm          algorithmic GEMM FLOPs (4 M n^2 T) over the two GEMMs' time,
                 against the dense MFMA peak of their operand type
   cpu_baseline  the FP64 CPU oracle (C++, OpenMP on all host cores) on the
@@ -275,6 +281,30 @@ def measured_traffic(workload, prec):
         return json.load(f).get("traffic_bytes")
 
 
+def valu_roofline(workload, prec, avg_ms):
+    """The E-step's VALU-issue bound: the SIMD cycles its VALU instructions
+    occupy per launch (rocprofv3 SQ_ACTIVE_INST_VALU quad-cycles x 4, kept in
+    profiles/ as JSON) over the SIMD cycles the GPU offers at its maximum
+    engine clock during the live average launch time."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"valu_{workload}_{PREC_NAME[prec]}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        v = json.load(f)
+    cyc = v["valu_issue_cycles"]
+    peak = v["simds"] * v["max_engine_clock_mhz"] * 1e6      # SIMD cycles per second
+    achieved = cyc / (avg_ms * 1e-3)
+    c = v["counters"]
+    return {"bound": "valu-issue", "kernel": "estep_kernel", "achieved": achieved / 1e9,
+            "peak": peak / 1e9, "unit": "G SIMD-cycles/s", "frac": achieved / peak,
+            "issue_cycles_per_launch": cyc,
+            "valu_insts_per_launch": c["SQ_INSTS_VALU"],
+            "transcendental_insts_per_launch": c["SQ_INSTS_VALU_TRANS_F32"],
+            "frac_under_counters": v["valu_issue_frac_under_counters"],
+            "clock_mhz_under_counters": v["clock_mhz_under_counters"]}
+
+
 def bench_batch(args):
     """Many recordings per EM iteration (one hipGraph): the batch workload of
     32 recordings, n in {32, 64, 128}, T ~ U[100k, 1.5M], M = 1, m = 3."""
@@ -432,6 +462,7 @@ def main():
                      "avg_ms": kms[2], "launches": cnt,
                      "measured_copy_gbs": hbm, "measured_read_gbs": hbm_read,
                      "frac_of_measured": achieved / hbm if hbm else None},
+        "valu": valu_roofline(w.name, prec, kms[2]),
         "gemm": {"flop_per_iter": gemm_flop, "ms": gemm_s * 1e3,
                  "tflops": gemm_flop / gemm_s / 1e12,
                  "peak_tflops": gemm_peak, "mfma_frac": gemm_flop / gemm_s / 1e12 / gemm_peak},

@@ -35,3 +35,19 @@ def test_amari_index_is_zero_exactly_for_scaled_permutations():
     W = P @ np.linalg.inv(A)
     assert bench._amari(W, A) < 1e-12
     assert bench._amari(np.linalg.inv(A + 0.1 * rng.normal(size=(6, 6))), A) > 1e-3
+
+
+def test_valu_roofline_from_the_committed_counters():
+    """The valu block recomputes its fractions from the committed counter
+    profile: 4 x SQ_ACTIVE_INST_VALU SIMD cycles per launch over 1024 SIMDs
+    at 2400 MHz times the launch time."""
+    import json
+    v = bench.valu_roofline("n128_T2M_M4", 16, 3.0)
+    prof = json.load(open(os.path.join(ROOT, "profiles", "valu_n128_T2M_M4_bf16.json")))
+    cyc = 4 * prof["counters"]["SQ_ACTIVE_INST_VALU"]
+    assert v["issue_cycles_per_launch"] == cyc
+    assert abs(v["frac"] - cyc / (1024 * 2.4e9 * 3.0e-3)) < 1e-12
+    grbm_per_xcd = prof["counters"]["GRBM_GUI_ACTIVE"] / 8
+    assert abs(v["frac_under_counters"] - cyc / 1024 / grbm_per_xcd) < 1e-12
+    assert 0.0 < v["frac_under_counters"] <= 1.0
+    assert bench.valu_roofline("n128_T2M_M4", 8, 3.0) is None      # no profile for that path
