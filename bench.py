@@ -58,15 +58,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-PRECISIONS = {"fp32": 32, "bf16": 16, "fp8": 8}
+PRECISIONS = {"fp32": 32, "fp32plain": 33, "bf16": 16, "fp8": 8}
 PREC_NAME = {v: k for k, v in PRECISIONS.items()}
 # arithmetic types of the path: GEMM operands + density math
-DTYPE = {32: "f32", 16: "bf16+f32", 8: "fp8+bf16+f32"}
+DTYPE = {32: "f64+f32", 33: "f32", 16: "bf16+f32", 8: "fp8+bf16+f32"}
 
 
 def estep_bytes(n, T, M, precision):
     """Algorithmic HBM bytes of one E-step launch: M n T (b read + g written)."""
-    g_bytes = 4 if precision == 32 else 2
+    g_bytes = 4 if precision in (32, 33) else 2
     return M * n * T * (4 + g_bytes)
 
 
@@ -192,8 +192,9 @@ def accuracy_vs_fp64(w, x, st0, precision, T_acc=60000, iters=50):
 
 
 # dense MFMA peaks of the GEMMs' operand type (MI355X, TFLOP/s): split-BF16
-# and FP8 paths run bf16 MFMAs, the FP32 path FP64 (unmixing) and FP32
-GEMM_PEAK_TFLOPS = {16: 2500.0, 8: 2500.0, 32: 78.6}
+# and FP8 paths run bf16 MFMAs, the FP32 path FP64 (unmixing) and FP32, the
+# plain FP32 path FP32
+GEMM_PEAK_TFLOPS = {16: 2500.0, 8: 2500.0, 32: 78.6, 33: 157.3}
 
 
 def measured_hbm_gbs(mib=2048, reps=5):

@@ -8,7 +8,7 @@
  * to DEFAULT_UPDATE_RHO and leaves update_rho at 0 (config.c:146-153), so a
  * file without `update_rho` runs with rho fixed.  Keys this engine adds
  * (unknown to the reference parser, which ignores them): `precision`
- * (fp32 | bf16 | fp8), `seed` (initial-state PRNG), `jsonlogfile` (a JSON
+ * (fp32 | fp32plain | bf16 | fp8), `seed` (initial-state PRNG), `jsonlogfile` (a JSON
  * line per iteration), `ainitfile` / `cinitfile`
  * (float64 initial A and c; the reference declares these fields but forces
  * them to NULL, config.c:131-134).
@@ -133,7 +133,7 @@ error read_devices(parsedconfigs* pc) {
     return SUCCESS;
 }
 
-/* `precision`: fp32 | bf16 | fp8 (or 32 | 16 | 8). */
+/* `precision`: fp32 | fp32plain | bf16 | fp8 (or 32 | 33 | 16 | 8). */
 error read_precision(parsedconfigs* pc) {
     char* text = NULL;
     const error rc = getString(pc, (char*)"precision", &text);
@@ -146,7 +146,7 @@ error read_precision(parsedconfigs* pc) {
         return rc;
     }
     static const struct { const char* a; const char* b; natural bits; } kNames[] = {
-        {"fp32", "32", 32}, {"bf16", "16", 16}, {"fp8", "8", 8}};
+        {"fp32", "32", 32}, {"fp32plain", "33", 33}, {"bf16", "16", 16}, {"fp8", "8", 8}};
     error out = ERRORINVALIDPARAM;
     for (const auto& e : kNames)
         if (!strcmp(text, e.a) || !strcmp(text, e.b)) {
@@ -229,7 +229,8 @@ void printConfig(void) {
     fprintf(stdout, "fix_init %lu\n", currentConfig.fix_init);
     fprintf(stdout, "block_size %lu\n", currentConfig.block_size);
     fprintf(stdout, "precision %s\n", currentConfig.precision == 16 ? "bf16"
-                                      : currentConfig.precision == 8 ? "fp8" : "fp32");
+                                      : currentConfig.precision == 8 ? "fp8"
+                                      : currentConfig.precision == 33 ? "fp32plain" : "fp32");
     fprintf(stdout, "====================================\n\n");
 }
 

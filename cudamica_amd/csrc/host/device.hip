@@ -83,7 +83,8 @@ error initializeDevs(void) {
     amica_options_default(&o, (int)currentConfig.nchannels, (long)currentConfig.nsamples,
                           (int)currentConfig.nmmodels, (int)currentConfig.nsdm);
     o.precision = currentConfig.precision == 16 ? AMICA_PREC_BF16
-                  : currentConfig.precision == 8 ? AMICA_PREC_FP8 : AMICA_PREC_FP32;
+                  : currentConfig.precision == 8 ? AMICA_PREC_FP8
+                  : currentConfig.precision == 33 ? AMICA_PREC_FP32_PLAIN : AMICA_PREC_FP32;
     o.max_iter = currentConfig.maxiter > 0 ? (int)currentConfig.maxiter : 1;
     o.device = gpus[0].realdevice;
     o.do_newton = (int)currentConfig.do_newton;

@@ -72,6 +72,7 @@ struct amica_engine {
     amica_options o;
     int NP = 0, ldb = 0, CW = 0;
     int bf16 = 0;
+    int plain = 0;                 /* plain FP32 path: FP32 MFMA GEMMs, FP32 b and g */
     int cus = 0;
     hipStream_t st = nullptr;
     /* data */
@@ -108,6 +109,7 @@ struct amica_engine {
     double *A = nullptr, *c = nullptr, *alpha = nullptr, *mu = nullptr, *beta = nullptr,
            *rho = nullptr, *gm = nullptr, *W = nullptr, *ldet = nullptr, *lconst = nullptr;
     double* Wg64 = nullptr;
+    float* Wg32 = nullptr;
     double* wc64 = nullptr;
     uint16_t* Whi = nullptr;
     uint16_t* Wlo = nullptr;
@@ -183,7 +185,7 @@ struct amica_engine {
         a.rholrate = o.rholrate; a.rhomin = o.rhomin; a.rhomax = o.rhomax; a.maxdec = o.maxdec;
         a.A = A; a.c = c; a.alpha = alpha; a.mu = mu; a.beta = beta; a.rho = rho; a.gm = gm;
         a.W = W; a.ldet = ldet; a.lconst = lconst;
-        a.Wg64 = Wg64; a.wc64 = wc64; a.Whi = Whi; a.Wlo = Wlo; a.wc = wc;
+        a.Wg64 = Wg64; a.Wg32 = Wg32; a.wc64 = wc64; a.Whi = Whi; a.Wlo = Wlo; a.wc = wc;
         a.xscale = fp8 ? xscale : nullptr;
         a.cst = cst; a.cst64 = cst64;
         a.stats = stats; a.stride = stride;
@@ -231,6 +233,8 @@ struct amica_engine {
             if (fp8) LCHK(launch_unmix_fp8(xq, Whi, Wlo, wc, b, o.T, NP, ldb, cu_budget, st));
             else LCHK(launch_unmix_bf16x3(xw, xbf, Whi, Wlo, wc, b, o.T, NP, ldb, cu_budget, st));
         }
+        else if (plain)
+            LCHK(launch_unmix_f32(xw, Wg32, wc, b, o.T, NP, ldb, cu_budget, st));
         else
             LCHK(launch_unmix_f64(xw64, Wg64, wc64, b64, o.T, NP, ldb, cu_budget, st));
         if (with_timing) HIPCHK(hipEventRecord(ev[2], st));
@@ -301,8 +305,9 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     if (o.T < 2) return fail(AMICA_E_INVALID, "T must be >= 2");
     if (o.M < 1 || o.M > AMICA_MAX_M) return fail(AMICA_E_INVALID, "M must be in [1, 16]");
     if (o.m < 1 || o.m > AMICA_MAX_MJ) return fail(AMICA_E_INVALID, "m must be in [1, 8]");
-    if (o.precision != AMICA_PREC_FP32 && o.precision != AMICA_PREC_BF16 && o.precision != AMICA_PREC_FP8)
-        return fail(AMICA_E_INVALID, "precision must be 32 (fp32), 16 (bf16) or 8 (fp8)");
+    if (o.precision != AMICA_PREC_FP32 && o.precision != AMICA_PREC_FP32_PLAIN &&
+        o.precision != AMICA_PREC_BF16 && o.precision != AMICA_PREC_FP8)
+        return fail(AMICA_E_INVALID, "precision must be 32 (fp32), 33 (plain fp32), 16 (bf16) or 8 (fp8)");
     const int NP = ((o.n + 63) / 64) * 64;
     const int CW = NP / 64;
     int ndev = 0;
@@ -322,6 +327,7 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     if (e->split) estep_split_shape(o.M, CW, &e->sp_mg, &e->sp_cwg, &e->sp_ncg, &e->sp_groups);
     e->bf16 = (o.precision == AMICA_PREC_BF16 || o.precision == AMICA_PREC_FP8);
     e->fp8 = (o.precision == AMICA_PREC_FP8);
+    e->plain = (o.precision == AMICA_PREC_FP32_PLAIN);
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, o.device));
     e->cus = prop.multiProcessorCount;
@@ -346,6 +352,8 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
         ALLOC(e->b, Tp * e->ldb * sizeof(float));
     } else if (e->bf16) {
         ALLOC(e->xbf, 2 * (size_t)T * NP * sizeof(uint16_t));   /* hi, then lo */
+        ALLOC(e->b, Tp * e->ldb * sizeof(float));
+    } else if (e->plain) {
         ALLOC(e->b, Tp * e->ldb * sizeof(float));
     } else {
         ALLOC(e->xw64, (size_t)T * NP * sizeof(double));
@@ -378,7 +386,8 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
         amica_engine_destroy(e);
         return fail(AMICA_E_DEVICE, "M-step LDS configuration failed");
     }
-    const int wg_per_cu = estep_blocks_per_cu(m, !e->bf16, 64 * waves, M == 1 && !e->split, e->split != 0);
+    const int wg_per_cu = estep_blocks_per_cu(m, e->b64 != nullptr, 64 * waves, M == 1 && !e->split,
+                                              e->split != 0);
     if (wg_per_cu < 1) {
         amica_engine_destroy(e);
         return fail(AMICA_E_DEVICE, "E-step occupancy query failed");
@@ -408,6 +417,8 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
     if (e->bf16) {
         ALLOC(e->Whi, (size_t)e->ldb * NP * sizeof(uint16_t));
         ALLOC(e->Wlo, (size_t)e->ldb * NP * sizeof(uint16_t));
+    } else if (e->plain) {
+        ALLOC(e->Wg32, (size_t)e->ldb * NP * sizeof(float));
     } else {
         ALLOC(e->Wg64, (size_t)e->ldb * NP * sizeof(double));
         ALLOC(e->wc64, (size_t)e->ldb * sizeof(double));
@@ -451,7 +462,7 @@ int amica_engine_destroy(amica_engine* e) {
     if (e->graph) (void)hipGraphDestroy(e->graph);
     void* ptrs[] = {e->xraw, e->xw, e->xw64, e->xbf, e->x8, e->xq, e->xscale, e->b, e->b64, e->g, e->part, e->gxpart,
                     e->raw, e->sumtmp, e->gxtot, e->A, e->c, e->alpha, e->mu, e->beta, e->rho, e->gm, e->W,
-                    e->ldet, e->lconst, e->Wg64, e->wc64, e->Whi, e->Wlo, e->wc, e->cst,
+                    e->ldet, e->lconst, e->Wg64, e->Wg32, e->wc64, e->Whi, e->Wlo, e->wc, e->cst,
                     e->cst64, e->stats, e->sigma2, e->cnew, e->phis,
                     e->kappa, e->lambda, e->Bm, e->work, e->lu, e->bflag, e->newton, e->ds,
                     e->log_ll, e->log_lrate, e->log_vsum, e->log_bflag, e->log_newton,
@@ -824,6 +835,7 @@ int amica_engine_debug_buffer(amica_engine* e, const char* name, void* out, size
         {"gx", e->gxtot, ldb * NP * sizeof(double)},
         {"raw", e->raw, e->pstride * sizeof(double)},
         {"Wg64", e->Wg64, e->Wg64 ? ldb * NP * sizeof(double) : 0},
+        {"Wg32", e->Wg32, e->Wg32 ? ldb * NP * sizeof(float) : 0},
         {"wc64", e->wc64, e->wc64 ? ldb * sizeof(double) : 0},
         {"wc", e->wc, ldb * sizeof(float)},
         {"ldet", e->ldet, (size_t)e->o.M * sizeof(double)},

@@ -125,6 +125,9 @@ int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double
 /* b = x_hi W_hi + x_hi W_lo + x_lo W_hi (three MFMAs, ~2^-16 relative) with
  * W given as bf16 hi and lo, x as the split bf16 copy xbf ([T][NP] hi, then
  * [T][NP] lo) or, for NP > 256, float32 x split on the fly. */
+/* Plain FP32 unmixing b = x W^T - W c on FP32 MFMA (Wg: [ldb][NP] FP32). */
+int launch_unmix_f32(const float* x, const float* Wg, const float* wc, float* b, long T, int NP,
+                     int ldb, int cus, hipStream_t st);
 int launch_unmix_bf16x3(const float* x, const uint16_t* xbf, const uint16_t* Whi,
                         const uint16_t* Wlo, const float* wc, float* b, long T, int NP, int ldb,
                         int cus, hipStream_t st);
@@ -179,8 +182,9 @@ struct MstepArgs {
     /* parameters (FP64) */
     double *A, *c, *alpha, *mu, *beta, *rho, *gm, *W, *ldet, *lconst;
     /* GEMM copies of W (rows h*NP + i, columns k): FP64 path Wg64 + wc64,
-     * BF16 path Whi + Wlo (bf16) + wc */
+     * BF16 path Whi + Wlo (bf16) + wc, plain FP32 path Wg32 + wc */
     double* Wg64;          /* [ldb][NP] */
+    float* Wg32;           /* [ldb][NP], or null (not the plain FP32 path) */
     double* wc64;          /* [ldb] */
     const float* xscale;   /* [NP] per-channel FP8 scales folded into Whi/Wlo, or null */
     uint16_t* Whi;         /* [ldb][NP] */

@@ -502,6 +502,150 @@ __global__ __launch_bounds__(NT) void unmix_f64_lds_kernel(const double* __restr
     }
 }
 
+/* ---------------- plain FP32 unmixing (comparator path) ------------- */
+/* b = x W^T - W c with v_mfma_f32_16x16x4f32 on FP32 operands: the path
+ * whose GEMMs are plain FP32, against which the split-BF16 path is compared.
+ * Register-resident W as in the FP64 kernel above, with the reduction index
+ * permuted so that every load is 16 bytes: lane group q = l / 16 covers the
+ * contiguous inputs [q KS, (q + 1) KS) (KS = NP / 4), one per k-step, so a
+ * lane holds W[col][q KS + s] for its KS k-steps (CT column blocks of 16)
+ * and reads x[sample][q KS + s .. + 3] of the staged tile with one LDS load
+ * per 4 k-steps.  16-sample tiles of x come from HBM kUnmixStages tiles
+ * ahead, 16 bytes per lane, through a double-buffered LDS slot (rows padded
+ * by 16 bytes: the 16 rows of one lane group hit distinct banks); one
+ * barrier per tile.  Output: register r of lane l holds sample 4 (l/16) + r
+ * of column l % 16. */
+template <int CT, int KS>
+__global__ __launch_bounds__(512) void unmix_f32_lds_kernel(const float* __restrict__ x,
+                                                            const float* __restrict__ Wg,
+                                                            const float* __restrict__ wc,
+                                                            float* __restrict__ b, long T, int ldb,
+                                                            long ntiles) {
+    constexpr int NP = 4 * KS;
+    constexpr int ROWB = 4 * NP;                   /* bytes of one sample row */
+    constexpr int RS = ROWB + 16;
+    constexpr int SLOT = 16 * RS;
+    constexpr int CPR = ROWB / 16;
+    constexpr int CHUNKS = 16 * CPR;
+    constexpr int NCH = (CHUNKS + 511) / 512;
+    constexpr int D = kUnmixStages;
+    __shared__ __align__(16) unsigned char lds[2 * SLOT];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int q = lane >> 4;
+    const int c0 = (blockIdx.y * 8 + w) * 16 * CT;
+    const bool computes = c0 < ldb;                /* wave-uniform */
+    f32x4 wf[CT][KS / 4];
+    float bias[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const int col = c0 + c * 16 + (lane & 15);
+        bias[c] = computes ? wc[col] : 0.f;
+#pragma unroll
+        for (int s4 = 0; s4 < KS / 4; ++s4)
+            wf[c][s4] = computes ? *(const f32x4*)(Wg + (size_t)col * NP + q * KS + 4 * s4)
+                                 : (f32x4){0.f, 0.f, 0.f, 0.f};
+    }
+    const long first = blockIdx.x, stride = gridDim.x;
+    const long ntl = first < ntiles ? (ntiles - 1 - first) / stride + 1 : 0;   /* block-uniform */
+
+    auto gload = [&](long j, uint4 (&r)[NCH]) {
+        const long tile = first + j * stride;
+#pragma unroll
+        for (int u = 0; u < NCH; ++u) {
+            const int c = tid + u * 512;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (c < CHUNKS && j < ntl) {
+                const int srow = c / CPR, cc = c - srow * CPR;
+                const long t = tile * 16 + srow;
+                if (t < T) v = *(const uint4*)((const unsigned char*)x + (size_t)t * ROWB + cc * 16);
+            }
+            r[u] = v;
+        }
+    };
+    auto lwrite = [&](int slot, const uint4 (&r)[NCH]) {
+#pragma unroll
+        for (int u = 0; u < NCH; ++u) {
+            const int c = tid + u * 512;
+            if (c < CHUNKS) {
+                const int srow = c / CPR, cc = c - srow * CPR;
+                *(uint4*)(lds + slot * SLOT + srow * RS + cc * 16) = r[u];
+            }
+        }
+    };
+    auto compute = [&](int slot, long j) {
+        const long tile = first + j * stride;
+        const unsigned char* base = lds + slot * SLOT + (lane & 15) * RS + q * KS * 4;
+        f32x4 acc[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s4 = 0; s4 < KS / 4; ++s4) {
+            const f32x4 av = *(const f32x4*)(base + 16 * s4);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+#pragma unroll
+                for (int c = 0; c < CT; ++c)
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], wf[c][s4][e], acc[c], 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const int col = c0 + c * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long t = tile * 16 + q * 4 + r;
+                if (t < T) b[(size_t)t * ldb + col] = acc[c][r] - bias[c];
+            }
+        }
+    };
+
+    uint4 R[D][NCH];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gload(d, R[d]);
+    for (long j0 = 0; j0 < ntl; j0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            const long j = j0 + u;
+            if (j < ntl) {                         /* block-uniform */
+                lwrite((int)(j & 1), R[u]);
+                gload(j + D, R[u]);
+                __syncthreads();
+                if (computes) compute((int)(j & 1), j);
+            }
+        }
+    }
+}
+
+/* Plain FP32 unmixing for any NP (the inputs past 256): one wave per 16
+ * samples x 16 columns, both operands read from global memory (the W
+ * columns stay in the caches), the same permuted reduction index. */
+__global__ __launch_bounds__(256) void unmix_f32_kernel(const float* __restrict__ x,
+                                                        const float* __restrict__ Wg,
+                                                        const float* __restrict__ wc,
+                                                        float* __restrict__ b, long T, int NP,
+                                                        int ldb) {
+    const int lane = threadIdx.x & 63, q = lane >> 4;
+    const long t0 = (long)blockIdx.x * 16;
+    const int col = (blockIdx.y * 4 + (threadIdx.x >> 6)) * 16 + (lane & 15);
+    const int KS = NP / 4;
+    const long ta = t0 + (lane & 15);
+    const float* xr = x + (size_t)(ta < T ? ta : T - 1) * NP + q * KS;
+    const float* wr = Wg + (size_t)col * NP + q * KS;
+    f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int s4 = 0; s4 < KS; s4 += 4) {
+        const f32x4 av = *(const f32x4*)(xr + s4);
+        const f32x4 wv = *(const f32x4*)(wr + s4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], wv[e], acc, 0, 0, 0);
+    }
+    const float bias = wc[col];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const long t = t0 + q * 4 + r;
+        if (t < T) b[(size_t)t * ldb + col] = acc[r] - bias;
+    }
+}
+
 /* ---------------- statistic GEMM G = g^T x, FP32 --------------------- */
 /* FP32 statistic GEMM in slabs: workgroup = ldb/64 waves over one chunk of
  * samples (at most a few thousand, so the FP32 accumulation inside the MFMA
@@ -711,6 +855,41 @@ int launch_unmix_f64(const double* x, const double* Wg, const double* wc, double
     }
     dim3 grid((unsigned)((T + 63) / 64), ldb / 64);
     hipLaunchKernelGGL(unmix_f64_kernel, grid, dim3(256), 0, st, x, Wg, wc, b, T, NP, ldb);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_unmix_f32(const float* x, const float* Wg, const float* wc, float* b, long T, int NP,
+                     int ldb, int cus, hipStream_t st) {
+    if (NP % 64 || ldb % 64) return -1;
+    const long ntiles = (T + 15) / 16;
+    /* register-resident W: CT NP / 4 floats per lane (at most 128) */
+    const int CT = (NP == 64 || NP == 128) ? 4 : (NP == 192 || NP == 256) ? 2 : 0;
+    if (CT && ldb % (16 * CT) == 0) {
+        const int cols = 8 * 16 * CT;
+        const int groups = (ldb + cols - 1) / cols;
+#define AMICA_UNMIX_F32(C, K)                                                                   \
+        do {                                                                                    \
+            static int bpc = 0;                                                                 \
+            if (bpc == 0) {                                                                     \
+                int nb = 0;                                                                     \
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                              \
+                        &nb, unmix_f32_lds_kernel<C, K>, 512, 0) != hipSuccess || nb < 1)      \
+                    nb = 1;                                                                     \
+                bpc = nb;                                                                       \
+            }                                                                                   \
+            const long gx = std::max(1L, std::min<long>(ntiles, (long)cus * bpc / groups));     \
+            hipLaunchKernelGGL((unmix_f32_lds_kernel<C, K>), dim3((unsigned)gx, groups),        \
+                               dim3(512), 0, st, x, Wg, wc, b, T, ldb, ntiles);                \
+        } while (0)
+        if (NP == 64) AMICA_UNMIX_F32(4, 16);
+        else if (NP == 128) AMICA_UNMIX_F32(4, 32);
+        else if (NP == 192) AMICA_UNMIX_F32(2, 48);
+        else AMICA_UNMIX_F32(2, 64);
+#undef AMICA_UNMIX_F32
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    hipLaunchKernelGGL(unmix_f32_kernel, dim3((unsigned)ntiles, ldb / 64), dim3(256), 0, st, x, Wg,
+                       wc, b, T, NP, ldb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

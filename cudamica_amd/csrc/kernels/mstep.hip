@@ -156,6 +156,10 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
                     /* the unmixing GEMM used W~ = W_hi + W_lo */
                     s0 += g * ((double)bf16_bits_to_f32(a.Whi[w0 + l]) + (double)bf16_bits_to_f32(a.Wlo[w0 + l]));
                     s1 += g * ((double)bf16_bits_to_f32(a.Whi[w1 + l]) + (double)bf16_bits_to_f32(a.Wlo[w1 + l]));
+                } else if (a.Wg32) {
+                    /* the plain FP32 GEMM used W~ = fp32(W) */
+                    s0 += g * (double)a.Wg32[w0 + l];
+                    s1 += g * (double)a.Wg32[w1 + l];
                 } else {
                     s0 += g * a.Wg64[w0 + l];
                     s1 += g * a.Wg64[w1 + l];
@@ -164,10 +168,11 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
             s0 = wave_sum_f64(s0);
             s1 = wave_sum_f64(s1);
             if (lane == 0) {
-                const double wc0 = a.prec_bf16 ? (double)a.wc[(size_t)h * NP + k0] : a.wc64[(size_t)h * NP + k0];
+                const bool wcf = a.prec_bf16 || a.Wg32;   /* W~ c held in FP32 */
+                const double wc0 = wcf ? (double)a.wc[(size_t)h * NP + k0] : a.wc64[(size_t)h * NP + k0];
                 st[o.phi + i + (long)k0 * n] = s0 - gs * wc0;
                 if (k1 != k0) {
-                    const double wc1 = a.prec_bf16 ? (double)a.wc[(size_t)h * NP + k1] : a.wc64[(size_t)h * NP + k1];
+                    const double wc1 = wcf ? (double)a.wc[(size_t)h * NP + k1] : a.wc64[(size_t)h * NP + k1];
                     st[o.phi + i + (long)k1 * n] = s1 - gs * wc1;
                 }
             }
@@ -457,6 +462,8 @@ __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
                 split_w(a.xscale ? wv2 * (double)a.xscale[k] : wv2, hi, lo);
                 a.Whi[dst] = hi;
                 a.Wlo[dst] = lo;
+            } else if (a.Wg32) {
+                a.Wg32[dst] = (float)wv2;
             } else {
                 a.Wg64[dst] = wv2;
             }
@@ -474,7 +481,7 @@ __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
             cj[kCstRm2 * ks] = (float)(r - 2.0);
             cj[kCstCq2 * ks] = (float)(cq * 1.4426950408889634);
             cj[kCstRsb * ks] = (float)(r * sb);
-            if (!a.prec_bf16) {
+            if (a.cst64) {
                 double* dj = a.cst64 + ((size_t)h * kNumCst64 * m + j) * NP + i;
                 dj[kCst64Sb * ks] = sb;
                 dj[kCst64Nsbmu * ks] = -sb * mv;
@@ -485,7 +492,7 @@ __global__ __launch_bounds__(256) void prepare_tail_kernel(MstepArgs a) {
 
 /* W~ c_h, one wave per row i over grid (n/4, M), lanes over k: the W~ the
  * unmixing GEMM applies (W~ = split(W diag(s)) diag(1/s) on the BF16 and FP8
- * paths, W itself on the FP32 path). */
+ * paths, fp32(W) on the plain FP32 path, W itself on the FP32 path). */
 __global__ __launch_bounds__(256) void wc_kernel(MstepArgs a) {
     const int h = blockIdx.y, n = a.n, lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -500,12 +507,14 @@ __global__ __launch_bounds__(256) void wc_kernel(MstepArgs a) {
             uint16_t hi, lo;
             split_w(wr * sk, hi, lo);
             wr = joined_w(hi, lo) / sk;
+        } else if (a.Wg32) {
+            wr = (double)(float)wr;
         }
         s += wr * c[k];
     }
     s = wave_sum_f64(s);
     if (lane == 0) {
-        if (a.prec_bf16) a.wc[(size_t)h * a.NP + i] = (float)s;
+        if (a.prec_bf16 || a.Wg32) a.wc[(size_t)h * a.NP + i] = (float)s;
         else a.wc64[(size_t)h * a.NP + i] = s;
     }
 }

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libcudamica_amd.so")
 CLI_PATH = os.path.join(_HERE, "bin", "cudaamica")
 
 PREC_FP32 = 32
+PREC_FP32_PLAIN = 33      # FP32 MFMA GEMMs, FP32 b and g: the plain FP32 comparator
 PREC_BF16 = 16
 PREC_FP8 = 8
 

@@ -32,7 +32,11 @@ extern "C" {
 #define AMICA_E_DEVICE      (-10)   /* HIP runtime error */
 #define AMICA_E_STATE       (-11)   /* call out of order (e.g. no data loaded) */
 
-#define AMICA_PREC_FP32       32    /* FP32 GEMMs and density math */
+#define AMICA_PREC_FP32       32    /* FP32 density math on FP64 b (FP64 unmixing GEMM), FP32
+                                       statistic GEMM: the parity path */
+#define AMICA_PREC_FP32_PLAIN 33    /* both GEMMs on FP32 operands (FP32 MFMA), FP32 b and g,
+                                       FP32 density math: the plain FP32 path the mixed
+                                       precision ones are compared with */
 #define AMICA_PREC_BF16       16    /* BF16 GEMM operands, FP32 accumulation and density math */
 #define AMICA_PREC_FP8         8    /* as BF16, with the whitened data stored as FP8 e4m3 with
                                        per-channel scales (the algorithm then runs on the
@@ -45,7 +49,8 @@ typedef struct {
     long   T;                /* samples */
     int    M;                /* models */
     int    m;                /* mixture components per source (1..8) */
-    int    precision;        /* AMICA_PREC_FP32, AMICA_PREC_BF16 or AMICA_PREC_FP8 */
+    int    precision;        /* AMICA_PREC_FP32, AMICA_PREC_FP32_PLAIN, AMICA_PREC_BF16 or
+                                AMICA_PREC_FP8 */
     int    max_iter;         /* capacity of the per-iteration logs */
     int    device;           /* HIP device ordinal */
     /* EM options, same meaning and defaults as the reference (config.h:46-83) */

@@ -113,7 +113,7 @@ typedef struct config_t {
     natural maxdec;
 
     /* additions of this engine */
-    natural precision;      /* key `precision`: fp32 (32), bf16 (16) or fp8 (8) */
+    natural precision;      /* key `precision`: fp32 (32), fp32plain (33), bf16 (16) or fp8 (8) */
     natural seed;           /* key `seed`: PRNG seed of the initial state */
     char*   jsonlogfile;    /* key `jsonlogfile`: one JSON line per iteration, or NULL */
 } config;

@@ -16,6 +16,7 @@ def test_estep_bytes_match_the_worked_values():
     assert bench.estep_bytes(64, 1_000_000, 1, 32) == 512_000_000
     assert bench.estep_bytes(128, 2_000_000, 4, 32) == 8_192_000_000
     assert bench.estep_bytes(128, 2_000_000, 4, 16) == 6_144_000_000
+    assert bench.estep_bytes(128, 2_000_000, 4, 33) == 8_192_000_000     # plain FP32: b and g FP32
     assert bench.estep_bytes(256, 4_000_000, 2, 32) == 16_384_000_000
     assert bench.estep_bytes(256, 4_000_000, 2, 8) == 12_288_000_000
 
