@@ -30,6 +30,9 @@ Prints one JSON line.  Fields beyond the driver's contract:
                 model the Amari distance between the GPU's and the
                 oracle's W, the relative errors of alpha, mu, beta, rho,
                 and each run's Amari distance to the true mixing matrices
+  plain_fp32    BF16 / FP8 runs: the same workload on the plain FP32 path (both
+                GEMMs on FP32 MFMA, FP32 b and g), 5 iterations after 2, and
+                speedup_vs_plain_fp32 = value / its rate
   ll_vs_fp32_path     BF16 / FP8 runs: the relative LL difference to the FP32
                 path (FP64 unmixing, held to the oracle by the parity tests)
                 after 50 iterations on the whole recording
@@ -250,6 +253,26 @@ def ll_vs_fp32_path(w, x, st0, precision, iters=50):
             "ll_last": float(logs[0]["ll"][-1]), "ll_last_fp32": float(logs[1]["ll"][-1])}
 
 
+def plain_fp32_rate(w, x, st0, steps=5, warmup=2):
+    """The same workload on the plain FP32 path (both GEMMs on FP32 MFMA, FP32
+    b and g): EM iterations per second over `steps` iterations after
+    `warmup`, timed as the headline is."""
+    import cudamica_amd as ca
+    with ca.Engine(w.options(precision=ca.PREC_FP32_PLAIN, max_iter=warmup + steps)) as eng:
+        eng.load_raw(x)
+        eng.preprocess(w.do_sphere)
+        eng.set_state(st0)
+        eng.iterate(warmup)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.iterate(steps)
+        eng.sync()
+        dt = time.perf_counter() - t0
+    log(f"plain FP32 path: {dt * 1e3 / steps:.3f} ms/iter")
+    return {"value": steps / dt, "unit": "EM iters/s", "ms_per_step": dt * 1e3 / steps,
+            "steps": steps, "warmup": warmup}
+
+
 def copy_probe(mib):
     """One hipMemset and one device-to-device hipMemcpy of `mib` MiB (the HIP
     runtime's fill and copy kernels), so that a rocprofv3 --pmc pass over this
@@ -375,6 +398,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the 50-iteration error report against the FP64 oracle")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip timing the plain FP32 path on the same workload")
     ap.add_argument("--no-ll-check", action="store_true",
                     help="skip the 50-iteration LL comparison with the FP32 path")
     ap.add_argument("--cu-budget", type=int, default=0, metavar="CUS",
@@ -471,6 +496,10 @@ def main():
                       "reduce_mstep": kms[4]},
         "ll_last": float(lg["ll"][-1]),
     }
+    if prec in (16, 8) and not args.no_plain:
+        pl = plain_fp32_rate(w, x, st0)
+        out["plain_fp32"] = pl
+        out["speedup_vs_plain_fp32"] = its / pl["value"]
     if prec != 32 and not args.no_ll_check:
         out["ll_vs_fp32_path"] = ll_vs_fp32_path(w, x, st0, prec)
     if not args.no_accuracy:
