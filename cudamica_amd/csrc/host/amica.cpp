@@ -126,6 +126,10 @@ extern "C" error runamica(void) {
     }
 
     std::vector<double> ll(maxiter, 0.0);
+    /* the E-step's algorithmic bytes per iteration, M n T (b read + g
+     * written): b is FP64 on the FP32 path, g bf16 on the BF16 and FP8 paths */
+    const natural prec = currentConfig.precision;
+    const double estep_bytes = (double)M * n * T * ((prec == 32 ? 8 : 4) + (prec == 16 || prec == 8 ? 2 : 4));
     std::vector<double> vs(M);
     std::vector<int> bfs(M), nws(M);
     FILE* jsonlog = nullptr;
@@ -138,10 +142,14 @@ extern "C" error runamica(void) {
         const clock_t c0 = clock();
         const auto t0 = std::chrono::steady_clock::now();
         /* one captured hipGraph per iteration; the last iteration records the
-         * per-sample L_h when ltall is written (not part of the graph) */
+         * per-sample L_h when ltall is written (not part of the graph), and
+         * with a JSON log every iteration runs eagerly between HIP events,
+         * which time its stages */
         const bool last_ltall = iter == maxiter && currentConfig.ltallfile;
         if (last_ltall) amica_engine_record_ltall(eng, 1);
-        const int irc = last_ltall ? amica_engine_iterate(eng, 1) : amica_engine_iterate_graph(eng, 1);
+        if (jsonlog) amica_engine_set_timing(eng, 1);     /* this iteration's events only */
+        const int irc = (last_ltall || jsonlog) ? amica_engine_iterate(eng, 1)
+                                                : amica_engine_iterate_graph(eng, 1);
         if (irc != 0 || amica_engine_sync(eng) != 0) {
             fprintf(stderr, "ERROR: iteration %lu: %s\n", iter, amica_last_error());
             exit(1);
@@ -167,8 +175,17 @@ extern "C" error runamica(void) {
                 (unsigned long)(clock() - c0), (unsigned long)msd);
         fflush(stdout);
         if (jsonlog) {
+            double sms[5] = {0, 0, 0, 0, 0};
+            int cnt = 0;
+            if (amica_engine_get_timing(eng, sms, &cnt) != 0) {
+                fprintf(stderr, "ERROR: %s\n", amica_last_error());
+                exit(1);
+            }
             fprintf(jsonlog, "{\"iteration\": %lu, \"ll\": %.17g, \"lrate\": %.17g, \"ll_decrease\": %d, "
-                             "\"ms\": %.4f, \"models\": [", iter, lli, lri, deci, msd);
+                             "\"ms\": %.4f, \"stages_ms\": {\"inverse\": %.4f, \"unmix\": %.4f, "
+                             "\"estep\": %.4f, \"stat_gemm\": %.4f, \"reduce_mstep\": %.4f}, "
+                             "\"estep_bytes\": %.0f, \"models\": [",
+                    iter, lli, lri, deci, msd, sms[0], sms[1], sms[2], sms[3], sms[4], estep_bytes);
             for (natural h = 0; h < M; ++h)
                 fprintf(jsonlog, "%s{\"vsum\": %.17g, \"bflag\": %d, \"newton\": %d}", h ? ", " : "", vs[h],
                         bfs[h], nws[h]);

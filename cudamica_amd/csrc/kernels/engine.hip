@@ -15,6 +15,7 @@
  * on the device, so an iteration can be replayed as one hipGraph.
  */
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <array>
@@ -65,6 +66,15 @@ namespace {
     } while (0)
 
 long canonical_stride(int n, int m) { return 1 + 8L * n * m + 2L * n + (long)n * n; }
+
+/* A roctx range over the host code that enqueues one EM stage (rocprofv3
+ * --marker-trace shows it and attributes the stage's kernels to it). */
+struct StageRange {
+    explicit StageRange(const char* name) { roctxRangePushA(name); }
+    ~StageRange() { roctxRangePop(); }
+    StageRange(const StageRange&) = delete;
+    StageRange& operator=(const StageRange&) = delete;
+};
 
 }  // namespace
 
@@ -225,9 +235,15 @@ struct amica_engine {
     /* prepare, unmixing GEMM, E-step, statistic GEMM, reductions */
     int enqueue_estep(bool with_timing) {
         if (with_timing) HIPCHK(hipEventRecord(ev[0], st));
-        int rc = enqueue_prepare();
+        int rc;
+        {
+            StageRange r("amica: inverse + GEMM copies of W");
+            rc = enqueue_prepare();
+        }
         if (rc) return rc;
         if (with_timing) HIPCHK(hipEventRecord(ev[1], st));
+        {
+        StageRange r("amica: unmixing GEMM");
         if (bf16)
         {
             if (fp8) LCHK(launch_unmix_fp8(xq, Whi, Wlo, wc, b, o.T, NP, ldb, cu_budget, st));
@@ -237,14 +253,22 @@ struct amica_engine {
             LCHK(launch_unmix_f32(xw, Wg32, wc, b, o.T, NP, ldb, cu_budget, st));
         else
             LCHK(launch_unmix_f64(xw64, Wg64, wc64, b64, o.T, NP, ldb, cu_budget, st));
+        }
         if (with_timing) HIPCHK(hipEventRecord(ev[2], st));
-        LCHK(launch_estep(eargs(ltall_on ? ltall : nullptr), o.m, nblocks, st));
+        {
+            StageRange r("amica: E-step");
+            LCHK(launch_estep(eargs(ltall_on ? ltall : nullptr), o.m, nblocks, st));
+        }
         if (with_timing) HIPCHK(hipEventRecord(ev[3], st));
-        if (bf16)
-            LCHK(launch_gx_bf16((const uint16_t*)g, x8, gxpart, o.T, NP, ldb, nchunks, st));
-        else
-            LCHK(launch_gx_f32((const float*)g, xw, gxpart, o.T, NP, ldb, nchunks, st));
+        {
+            StageRange r("amica: statistic GEMM");
+            if (bf16)
+                LCHK(launch_gx_bf16((const uint16_t*)g, x8, gxpart, o.T, NP, ldb, nchunks, st));
+            else
+                LCHK(launch_gx_f32((const float*)g, xw, gxpart, o.T, NP, ldb, nchunks, st));
+        }
         if (with_timing) HIPCHK(hipEventRecord(ev[4], st));
+        StageRange r("amica: reductions");
         return enqueue_reduce(nblocks, nchunks);
     }
 
@@ -256,11 +280,13 @@ struct amica_engine {
     }
 
     int enqueue_mstep() {
+        StageRange r("amica: M-step");
         LCHK(launch_mstep(margs(), st));
         return 0;
     }
 
     int enqueue_iteration(bool with_timing) {
+        StageRange r("amica: EM iteration");
         int rc = enqueue_estep(with_timing);
         if (rc) return rc;
         rc = enqueue_mstep();

@@ -2,9 +2,11 @@
 reference's stdout lines and raw float64 result files out.  The files must
 be exactly what the engine's C ABI computes from the same inputs, and the
 LL trajectory must follow the CPU oracle on the same data and initial
-state.  The program replays one captured hipGraph per iteration, reads
-only that iteration's log entry, and writes one JSON line per iteration
-(`jsonlogfile`)."""
+state.  The program replays one captured hipGraph per iteration and reads
+only that iteration's log entry; with `jsonlogfile` it runs each iteration
+eagerly between HIP events and writes one JSON line per iteration (LL,
+lrate, per-model vsum / bflag / Newton flag, stage times, E-step bytes);
+both ways give the same trajectory bit for bit."""
 import os
 import re
 import subprocess
@@ -77,6 +79,19 @@ def test_cli_end_to_end(built, tmp_path):
     np.testing.assert_array_equal([j["lrate"] for j in js], elog["lrate"])
     np.testing.assert_array_equal([[mm["vsum"] for mm in j["models"]] for j in js], elog["vsum"])
     assert all(j["ms"] > 0 for j in js)
+    stages = ("inverse", "unmix", "estep", "stat_gemm", "reduce_mstep")
+    assert all(j["stages_ms"][k] > 0 for j in js for k in stages)
+    assert all(sum(j["stages_ms"].values()) <= j["ms"] for j in js)
+    assert all(j["estep_bytes"] == M * n * T * (8 + 4) for j in js)     # FP32 path: b FP64, g FP32
+    # without the JSON log every iteration replays the captured graph: the
+    # same trajectory, bit for bit
+    with open(os.path.join(d, "run_graph.param"), "w") as f:
+        f.write("\n".join(l for l in lines if not l.startswith("jsonlogfile")) + "\n")
+    r2 = subprocess.run([ca.CLI_PATH, "-f", os.path.join(d, "run_graph.param")], capture_output=True,
+                        text=True, timeout=180)
+    assert r2.returncode == 0, r2.stdout[-3000:] + r2.stderr[-3000:]
+    got2 = re.findall(r"Iteration (\d+): lrate ([-0-9.e+]+) LL ([-0-9.e+]+)", r2.stdout)
+    assert got2 == re.findall(r"Iteration (\d+): lrate ([-0-9.e+]+) LL ([-0-9.e+]+)", r.stdout)
     np.testing.assert_allclose(rd("weightsfile", n * n * M), est["W"], rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(rd("afile", n * n * M), Aout, rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(rd("cfile", n * M), cout, rtol=1e-12, atol=1e-14)
