@@ -30,6 +30,9 @@ Prints one JSON line.  Fields beyond the driver's contract:
                 model the Amari distance between the GPU's and the
                 oracle's W, the relative errors of alpha, mu, beta, rho,
                 and each run's Amari distance to the true mixing matrices
+  adjoint       amica_ll_forward / amica_ll_adjoint (FP32 density math) on the
+                whole whitened recording at the state the iterations reached:
+                milliseconds per call
   plain_fp32    BF16 / FP8 runs: the same workload on the plain FP32 path (both
                 GEMMs on FP32 MFMA, FP32 b and g), 5 iterations after 2, and
                 speedup_vs_plain_fp32 = value / its rate
@@ -273,6 +276,46 @@ def plain_fp32_rate(w, x, st0, steps=5, warmup=2):
             "steps": steps, "warmup": warmup}
 
 
+def adjoint_rate(xw, ldetS, st, n, M, m, reps=3):
+    """amica_ll_forward and amica_ll_adjoint (real_bits 32: FP32 density math,
+    FP64 sums and gradients) over the whole whitened recording at the state
+    the timed iterations reached, dll = 1: median milliseconds per call of
+    `reps` after one untimed call.  Device buffers come from torch."""
+    import torch
+    import cudamica_amd as ca
+    dev = torch.device("cuda:0")
+    T = xw.shape[0]
+    x = torch.from_numpy(np.ascontiguousarray(xw, np.float32)).to(dev)
+    ps = {k: torch.tensor(np.asarray(st[k], np.float64).ravel(), device=dev) for k in ca.PARAM_NAMES}
+    gs = {k: torch.empty_like(p) for k, p in ps.items()}
+    ll = torch.empty(T, dtype=torch.float64, device=dev)
+    dll = torch.ones(T, dtype=torch.float64, device=dev)
+    pa = {k: p.data_ptr() for k, p in ps.items()}
+    ga = {k: g.data_ptr() for k, g in gs.items()}
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def fwd():
+        ca.ll_forward(32, x.data_ptr(), T, n, M, m, pa, ldetS, ll.data_ptr(), stream)
+
+    def adj():
+        ca.ll_adjoint(32, x.data_ptr(), T, n, M, m, pa, ldetS, dll.data_ptr(), ga, 0, stream)
+
+    out = {}
+    for name, fn in (("forward_ms", fwd), ("adjoint_ms", adj)):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()                      # each call synchronises its stream
+            ts.append((time.perf_counter() - t0) * 1e3)
+        out[name] = float(np.median(ts))
+    log(f"ll forward {out['forward_ms']:.2f} ms, adjoint {out['adjoint_ms']:.2f} ms per call")
+    return {"real_bits": 32, "samples": T, **out,
+            "note": "per call on the whole recording, workspace allocation and synchronisation "
+                    "included; gradients w.r.t. W, c, alpha, mu, beta, rho, gm with dll = 1"}
+
+
 def copy_probe(mib):
     """One hipMemset and one device-to-device hipMemcpy of `mib` MiB (the HIP
     runtime's fill and copy kernels), so that a rocprofv3 --pmc pass over this
@@ -398,6 +441,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the 50-iteration error report against the FP64 oracle")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip timing the likelihood and its adjoint on the recording")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip timing the plain FP32 path on the same workload")
     ap.add_argument("--no-ll-check", action="store_true",
@@ -415,6 +460,11 @@ def main():
     if args.workload == "batch32":
         bench_batch(args)
         return
+    if not args.no_adjoint:
+        # torch provides the adjoint timing's device buffers; it must bring up
+        # its HIP context before this library's first HIP call
+        import torch
+        torch.cuda.init()
     import cudamica_amd as ca
     w = ca.WORKLOADS[args.workload]
     if args.T:
@@ -454,6 +504,9 @@ def main():
     kms, cnt = eng.timing()
     eng.set_timing(False)
     lg = eng.log(args.warmup + args.steps + timing_steps)
+    adj_in = None
+    if not args.no_adjoint:
+        adj_in = (eng.white(), eng.sphere()[2], eng.get_state())
     eng.close()
     its = args.steps / dt
     log(f"timed {args.steps} iterations: {dt * 1e3 / args.steps:.3f} ms/iter; kernels (ms) "
@@ -496,6 +549,9 @@ def main():
                       "reduce_mstep": kms[4]},
         "ll_last": float(lg["ll"][-1]),
     }
+    if adj_in is not None:
+        out["adjoint"] = adjoint_rate(adj_in[0], adj_in[1], adj_in[2], n, M, m)
+        adj_in = None
     if prec in (16, 8) and not args.no_plain:
         pl = plain_fp32_rate(w, x, st0)
         out["plain_fp32"] = pl

@@ -17,12 +17,15 @@
  * with f = rho sign(y)|y|^(rho-1).
  *
  * Kernels: prep (per model: W^-1 by LU, log|det W|, W c, the constant part
- * of Q), ll (one wave per (model, 64-channel block) over a range of
- * samples, all M ceil(n/64) waves in one workgroup where they fit in 8, else
- * three passes through per-sample buffers; x tiles in LDS; b = W x - W c by
- * each lane for its channel),
- * gx (sum_t g_t x_t^T, 64 x 64 FP64 tiles over sample chunks), and finish
- * (fixed-order sums over workgroups and the assembly of the gradients).
+ * of Q), the unmixing GEMM b = W x - W c on MFMA (the engine's FP64 or plain
+ * FP32 kernels, into a [T][M NP] buffer), ll (one wave per (model, 64-channel
+ * block) over a range of samples, all M ceil(n/64) waves in one workgroup
+ * where they fit in 8, else three passes through per-sample buffers; b read
+ * from the GEMM's buffer -- the allocation-free forward instead forms it per
+ * lane from x tiles in LDS), gx (sum_t g_t x_t^T: FP32 MFMA on the FP32
+ * variant, 64 x 64 FP64 tiles on the FP64 one, over sample chunks), and
+ * finish (fixed-order sums over workgroups and the assembly of the
+ * gradients).
  * Every reduction is in a fixed order, so results repeat bit for bit.
  */
 #include <hip/hip_runtime.h>
@@ -66,6 +69,10 @@ struct AdjArgs {
     double* qbuf;
     double* vbuf;
     int pg;
+    /* b = W x - W c from the unmixing GEMM, [T][M NP] Real, or null: each
+     * lane then forms its b from an LDS tile of x (the allocation-free
+     * forward) */
+    const void* bbuf;
 };
 
 enum { kLlFused = 0, kLlQ = 1, kLlGrad = 2 };
@@ -73,14 +80,22 @@ enum { kLlFused = 0, kLlQ = 1, kLlGrad = 2 };
 /* per pair (h, i): dalpha_j, dmu_j, dbeta_j, drho_j (j < m), gsum; then Omega_h */
 __host__ __device__ inline int pair_stride(int m) { return 4 * m + 1; }
 
+/* Per model: W^-1 and log|det W| -- here by LU in this workgroup (the
+ * allocation-free forward, a.lu set), else already computed by the
+ * Gauss-Jordan launches, which leave -log|det W| in ldet -- then W c and
+ * the constant part of Q. */
 __global__ __launch_bounds__(256) void adj_prep_kernel(AdjArgs a) {
     __shared__ int perm[1024];
     const int h = blockIdx.x, n = a.n, m = a.m;
     const long nn = (long)n * n;
     const double* W = a.W + (size_t)h * nn;
-    const double lad = block_lu_inverse(W, n, a.lu + (size_t)h * nn, a.X + (size_t)h * nn,
-                                        a.Winv + (size_t)h * nn, perm);
-    if (threadIdx.x == 0) a.ldet[h] = lad;
+    if (a.lu) {
+        const double lad = block_lu_inverse(W, n, a.lu + (size_t)h * nn, a.X + (size_t)h * nn,
+                                            a.Winv + (size_t)h * nn, perm);
+        if (threadIdx.x == 0) a.ldet[h] = lad;
+    } else if (threadIdx.x == 0) {
+        a.ldet[h] = -a.ldet[h];
+    }
     const double* c = a.c + (size_t)h * n;
     for (int i = threadIdx.x; i < a.NP; i += blockDim.x) {
         double s = 0.0;
@@ -95,12 +110,21 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(AdjArgs a) {
     }
 }
 
+/* FP32: the hardware base-2 logarithm and exponential (v_log_f32,
+ * v_exp_f32; arguments here are normal or underflow to 0) and reciprocal. */
 template <typename Real> __device__ __forceinline__ Real r_log(Real v);
-template <> __device__ __forceinline__ float r_log<float>(float v) { return logf(v); }
+template <> __device__ __forceinline__ float r_log<float>(float v) {
+    return __builtin_amdgcn_logf(v) * 0.6931471805599453f;
+}
 template <> __device__ __forceinline__ double r_log<double>(double v) { return log(v); }
 template <typename Real> __device__ __forceinline__ Real r_exp(Real v);
-template <> __device__ __forceinline__ float r_exp<float>(float v) { return expf(v); }
+template <> __device__ __forceinline__ float r_exp<float>(float v) {
+    return __builtin_amdgcn_exp2f(v * 1.4426950408889634f);
+}
 template <> __device__ __forceinline__ double r_exp<double>(double v) { return exp(v); }
+template <typename Real> __device__ __forceinline__ Real r_rcp(Real v);
+template <> __device__ __forceinline__ float r_rcp<float>(float v) { return __builtin_amdgcn_rcpf(v); }
+template <> __device__ __forceinline__ double r_rcp<double>(double v) { return 1.0 / v; }
 
 template <typename Real> struct RealTiny;
 template <> struct RealTiny<float> { static constexpr float v = 1.17549435e-38f; };
@@ -154,7 +178,7 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
     double* vh = Lh + S * M;
 
     Real sb[MJ], mu[MJ], rho[MJ], cq[MJ];
-    double psi1[ADJ ? MJ : 1], al[ADJ ? MJ : 1], be[ADJ ? MJ : 1];
+    double psi1[ADJ ? MJ : 1], al[ADJ ? MJ : 1], be[ADJ ? MJ : 1], sbd[ADJ ? MJ : 1], rhod[ADJ ? MJ : 1];
 #pragma unroll
     for (int j = 0; j < MJ; ++j) {
         const size_t e = ((size_t)h * MJ + j) * n + (act ? i : 0);
@@ -166,36 +190,50 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
             psi1[j] = psi_dev(1.0 + 1.0 / a.rho[e]);
             al[j] = a.alpha[e];
             be[j] = a.beta[e];
+            sbd[j] = sqrt(a.beta[e]);
+            rhod[j] = a.rho[e];
         }
     }
     const Real wci = (Real)a.wc[(size_t)h * NP + i];
     const double* Wh = a.W + (size_t)h * n * n;
-    double ga[ADJ ? MJ : 1], gmu[ADJ ? MJ : 1], gb[ADJ ? MJ : 1], gr[ADJ ? MJ : 1];
+    /* per component: sum u, sum u f, sum u f y, sum u |y|^rho ln|y|; the
+     * gradients are these times per-(h, j, i) constants, applied at the end */
+    double su[ADJ ? MJ : 1], suf[ADJ ? MJ : 1], sufy[ADJ ? MJ : 1], surl[ADJ ? MJ : 1];
     double gsum = 0.0, om = 0.0;
 #pragma unroll
-    for (int j = 0; j < (ADJ ? MJ : 1); ++j) ga[j] = gmu[j] = gb[j] = gr[j] = 0.0;
+    for (int j = 0; j < (ADJ ? MJ : 1); ++j) su[j] = suf[j] = sufy[j] = surl[j] = 0.0;
 
     const long t0 = (long)blockIdx.x * a.chunk;
     const long t1 = min(a.T, t0 + a.chunk);
     const Real* x = (const Real*)a.x;
+    const Real* const bb = (const Real*)a.bbuf;
+    const size_t bcol = (size_t)h * NP + i, ldb = (size_t)M * NP;
     for (long tb = t0; tb < t1; tb += S) {
-        for (int q = tid; q < S * NP; q += blockDim.x) {
-            const int s = q / NP, k = q - (q / NP) * NP;
-            const long t = tb + s;
-            xs[q] = (t < t1 && k < n) ? x[(size_t)t * n + k] : (Real)0;
-        }
-        __syncthreads();
         Real bv[S];
+        if (bb) {                                    /* block-uniform */
 #pragma unroll
-        for (int s = 0; s < S; ++s) bv[s] = (Real)0;
-        if (act)
-            for (int k = 0; k < n; ++k) {
-                const Real wik = (Real)Wh[i + (long)k * n];
-#pragma unroll
-                for (int s = 0; s < S; ++s) bv[s] = fma(wik, xs[s * NP + k], bv[s]);
+            for (int s = 0; s < S; ++s) {
+                const long t = tb + s;
+                bv[s] = (act && t < t1) ? bb[(size_t)t * ldb + bcol] : (Real)0;
             }
+        } else {
+            for (int q = tid; q < S * NP; q += blockDim.x) {
+                const int s = q / NP, k = q - (q / NP) * NP;
+                const long t = tb + s;
+                xs[q] = (t < t1 && k < n) ? x[(size_t)t * n + k] : (Real)0;
+            }
+            __syncthreads();
 #pragma unroll
-        for (int s = 0; s < S; ++s) bv[s] -= wci;
+            for (int s = 0; s < S; ++s) bv[s] = (Real)0;
+            if (act)
+                for (int k = 0; k < n; ++k) {
+                    const Real wik = (Real)Wh[i + (long)k * n];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) bv[s] = fma(wik, xs[s * NP + k], bv[s]);
+                }
+#pragma unroll
+            for (int s = 0; s < S; ++s) bv[s] -= wci;
+        }
         if constexpr (MODE != kLlGrad) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
@@ -274,20 +312,19 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
                         ez[j] = r_exp<Real>(Qj[j] - mx);
                         se += ez[j];
                     }
+                    const Real rse = r_rcp<Real>(se);
 #pragma unroll
                     for (int j = 0; j < MJ; ++j) {
-                        const double u = omh * (double)(ez[j] / se);
-                        /* f y = rho |y|^rho and f = rho |y|^rho / y (y != 0) */
-                        const double fy = (double)rho[j] * (double)YRj[j];
-                        const double ay = fmax(fabs((double)Yj[j]), (double)RealTiny<Real>::v);
-                        const double f = (Yj[j] == (Real)0) ? 0.0 : copysign(fy / ay, (double)Yj[j]);
-                        const double sbj = sqrt(be[j]);
-                        ga[j] += u / al[j];
-                        gmu[j] += sbj * u * f;
-                        gb[j] += u * (1.0 - fy) / (2.0 * be[j]);
-                        gr[j] += u * (psi1[j] / ((double)rho[j] * (double)rho[j]) -
-                                      (double)YRj[j] * (double)LNj[j]);
-                        gv += sbj * u * f;
+                        const double u = omh * (double)(ez[j] * rse);
+                        /* f y = rho |y|^rho and f = f y / y (0 at y = 0) */
+                        const Real fy = rho[j] * YRj[j];
+                        const Real f = (Yj[j] == (Real)0) ? (Real)0 : fy * r_rcp<Real>(Yj[j]);
+                        const double uf = u * (double)f;
+                        su[j] += u;
+                        suf[j] += uf;
+                        sufy[j] += u * (double)fy;
+                        surl[j] += u * ((double)YRj[j] * (double)LNj[j]);
+                        gv += sbd[j] * uf;
                     }
                 }
                 if (wreal) g[(size_t)t * ldg + (size_t)h * NP + i] = (Real)gv;
@@ -301,12 +338,15 @@ __global__ __launch_bounds__(512) void ll_kernel(AdjArgs a) {
         if (!wreal) return;
         double* base = a.part + (size_t)blockIdx.x * a.pstride;
         double* pp = base + ((size_t)h * NP + i) * pair_stride(MJ);
+        /* dalpha = sum u / alpha, dmu = sqrt(beta) sum u f,
+         * dbeta = sum u (1 - f y) / (2 beta),
+         * drho = sum u (psi(1 + 1/rho) / rho^2 - |y|^rho ln|y|) */
 #pragma unroll
         for (int j = 0; j < MJ; ++j) {
-            pp[j] = ga[j];
-            pp[MJ + j] = gmu[j];
-            pp[2 * MJ + j] = gb[j];
-            pp[3 * MJ + j] = gr[j];
+            pp[j] = su[j] / al[j];
+            pp[MJ + j] = sbd[j] * suf[j];
+            pp[2 * MJ + j] = (su[j] - sufy[j]) / (2.0 * be[j]);
+            pp[3 * MJ + j] = su[j] * psi1[j] / (rhod[j] * rhod[j]) - surl[j];
         }
         pp[4 * MJ] = gsum;
         if (i == 0) base[(size_t)M * NP * pair_stride(MJ) + h] = om;
@@ -397,6 +437,46 @@ __global__ __launch_bounds__(256) void adj_gx_kernel(const Real* __restrict__ g,
         }
 }
 
+/* The unmixing GEMM's operands: Wg[h NP + i][k] = W_h[i][k] (zero past n)
+ * and W c in the GEMM's precision. */
+template <typename Real>
+__global__ __launch_bounds__(256) void adj_wg_kernel(AdjArgs a, Real* __restrict__ Wg,
+                                                     Real* __restrict__ wcr) {
+    const int NP = a.NP, n = a.n;
+    const long total = (long)a.M * NP * NP;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        const long r = q / NP;
+        const int k = (int)(q - r * NP), h = (int)(r / NP), i = (int)(r - (long)h * NP);
+        Wg[q] = (i < n && k < n) ? (Real)a.W[(size_t)h * n * n + (size_t)k * n + i] : (Real)0;
+        if (k == 0) wcr[r] = (Real)a.wc[r];
+    }
+}
+
+/* x [T][n] into [T][NP] with zero columns past n (the GEMMs' row layout) */
+template <typename Real>
+__global__ __launch_bounds__(256) void adj_pad_kernel(const Real* __restrict__ x, Real* __restrict__ xp,
+                                                      long T, int n, int NP) {
+    const long total = T * NP;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        const long t = q / NP;
+        const int k = (int)(q - t * NP);
+        xp[q] = k < n ? x[(size_t)t * n + k] : (Real)0;
+    }
+}
+
+/* Fixed-order sums over slots: out[e] = sum_{b < nslots} in[b stride + e]
+ * for e < count, one thread per entry (consecutive entries coalesce). */
+__global__ __launch_bounds__(256) void adj_slot_sum_kernel(const double* __restrict__ in, size_t stride,
+                                                           int nslots, size_t count,
+                                                           double* __restrict__ out) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (size_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int b = 0; b < nslots; ++b) acc += in[(size_t)b * stride + e];
+        out[e] = acc;
+    }
+}
+
 struct AdjOut {
     double *dW, *dc, *dalpha, *dmu, *dbeta, *drho, *dgm;
 };
@@ -405,7 +485,7 @@ struct AdjOut {
  * gx chunks, then the gradients. */
 __global__ __launch_bounds__(256) void adj_finish_kernel(AdjArgs a, int nblocks,
                                                          const double* __restrict__ gxpart,
-                                                         int nchunks, AdjOut o) {
+                                                         int nchunks, int gxs, AdjOut o) {
     extern __shared__ double gs[];   /* gsum_h[n] */
     __shared__ double om_sh;
     const int h = blockIdx.x, n = a.n, m = a.m, NP = a.NP, M = a.M;
@@ -457,7 +537,7 @@ __global__ __launch_bounds__(256) void adj_finish_kernel(AdjArgs a, int nblocks,
     for (long q = threadIdx.x; q < nn; q += blockDim.x) {
         const int i = (int)(q % n), k = (int)(q / n);
         double G = 0.0;
-        for (int ch = 0; ch < nchunks; ++ch) G += gxpart[((size_t)ch * ldg + (size_t)h * NP + i) * n + k];
+        for (int ch = 0; ch < nchunks; ++ch) G += gxpart[((size_t)ch * ldg + (size_t)h * NP + i) * gxs + k];
         o.dW[(size_t)h * nn + q] = om * Wi[k + (long)i * n] - (G - gs[i] * c[k]);
     }
 }
@@ -501,7 +581,7 @@ int launch_ll(int MJ, const AdjArgs& a, int nblocks, size_t shm, hipStream_t st)
 
 /* Workspace of one call, freed on every exit path. */
 struct Workspace {
-    static constexpr int kMax = 12;
+    static constexpr int kMax = 20;
     void* ptrs[kMax] = {nullptr};
     int n = 0;
     ~Workspace() {
@@ -608,9 +688,16 @@ int ll_call(int real_bits, const void* x, long T, int n, int M, int m, const dou
         a.wc = a.cq + kNoAllocNM;
         a.ldet = a.wc + kNoAllocNP;
     } else if (!ws.alloc((void**)&a.Winv, M * nn * 8) || !ws.alloc((void**)&a.ldet, M * 8) ||
-               !ws.alloc((void**)&a.cq, (size_t)M * m * n * 8) || !ws.alloc((void**)&a.wc, (size_t)M * NP * 8) ||
-               !ws.alloc((void**)&a.lu, M * nn * 8) || !ws.alloc((void**)&a.X, M * nn * 8)) {
+               !ws.alloc((void**)&a.cq, (size_t)M * m * n * 8) || !ws.alloc((void**)&a.wc, (size_t)M * NP * 8)) {
         return report_error(AMICA_E_NOMEM, "workspace allocation failed");
+    }
+    if (!noalloc) {
+        /* W^-1 and log|det W| by the M-step's blocked Gauss-Jordan */
+        double* gjw = nullptr;
+        if (!ws.alloc((void**)&gjw, (8 + M * nn + inverse_work_doubles(n, M)) * 8))
+            return report_error(AMICA_E_NOMEM, "workspace allocation failed");
+        if (launch_gj_inverse(W, a.Winv, a.ldet, n, M, gjw, st))
+            return report_error(AMICA_E_DEVICE, "inverse launch failed");
     }
     if (split) {
         a.pg = 8;
@@ -618,16 +705,59 @@ int ll_call(int real_bits, const void* x, long T, int n, int M, int m, const dou
             (adj && !ws.alloc((void**)&a.vbuf, (size_t)T * M * 8)))
             return report_error(AMICA_E_NOMEM, "workspace allocation failed");
     }
-    const int gchunks = (int)std::max(1L, std::min((long)cus, (T + 255) / 256));
+    const int ldb = M * NP;
+    /* the statistic GEMM sum_t g_t x_t^T: FP32 MFMA over padded rows of x
+     * (gx_f32_slab_kernel, row stride NP) on the FP32 variant, FP64 tiles
+     * over the unpadded rows (row stride n) on the FP64 one */
+    const bool gx_mfma = adj && real_bits == 32;
+    const int gchunks = gx_mfma ? gx_f32_chunks(T, NP, ldb, cus)
+                                : (int)std::max(1L, std::min((long)cus, (T + 255) / 256));
+    const int gxs = gx_mfma ? NP : n;
     double* gxpart = nullptr;
     if (adj) {
-        if (!ws.alloc(&a.g, (size_t)T * M * NP * rb) ||
+        if (!ws.alloc(&a.g, (size_t)T * ldb * rb) ||
             !ws.alloc((void**)&a.part, (size_t)nb * a.pstride * 8) ||
-            !ws.alloc((void**)&gxpart, (size_t)gchunks * M * NP * n * 8))
+            !ws.alloc((void**)&gxpart, (size_t)gchunks * ldb * gxs * 8))
             return report_error(AMICA_E_NOMEM, "workspace allocation failed");
     }
     hipLaunchKernelGGL(adj_prep_kernel, dim3(M), dim3(256), 0, st, a);
     if (hipGetLastError() != hipSuccess) return report_error(AMICA_E_DEVICE, "prep launch failed");
+    /* b = W x - W c by the unmixing GEMM on MFMA (FP64 or FP32 operands)
+     * into a [T][M NP] buffer; the allocation-free forward forms b per lane */
+    const void* xp = x;
+    if (!noalloc) {
+        void *Wg = nullptr, *wcr = nullptr, *bb = nullptr;
+        if (!ws.alloc(&Wg, (size_t)ldb * NP * rb) || !ws.alloc(&wcr, (size_t)ldb * rb) ||
+            !ws.alloc(&bb, (size_t)T * ldb * rb))
+            return report_error(AMICA_E_NOMEM, "workspace allocation failed");
+        const unsigned eg = (unsigned)std::min<long>(4096, ((long)ldb * NP + 255) / 256);
+        if (n != NP) {
+            void* xpad = nullptr;
+            if (!ws.alloc(&xpad, (size_t)T * NP * rb))
+                return report_error(AMICA_E_NOMEM, "workspace allocation failed");
+            const unsigned pg = (unsigned)std::min<long>(65536, (T * NP + 255) / 256);
+            if (real_bits == 64)
+                hipLaunchKernelGGL(adj_pad_kernel<double>, dim3(pg), dim3(256), 0, st, (const double*)x,
+                                   (double*)xpad, T, n, NP);
+            else
+                hipLaunchKernelGGL(adj_pad_kernel<float>, dim3(pg), dim3(256), 0, st, (const float*)x,
+                                   (float*)xpad, T, n, NP);
+            xp = xpad;
+        }
+        int urc;
+        if (real_bits == 64) {
+            hipLaunchKernelGGL(adj_wg_kernel<double>, dim3(eg), dim3(256), 0, st, a, (double*)Wg, (double*)wcr);
+            urc = launch_unmix_f64((const double*)xp, (const double*)Wg, (const double*)wcr, (double*)bb, T,
+                                   NP, ldb, cus, st);
+        } else {
+            hipLaunchKernelGGL(adj_wg_kernel<float>, dim3(eg), dim3(256), 0, st, a, (float*)Wg, (float*)wcr);
+            urc = launch_unmix_f32((const float*)xp, (const float*)Wg, (const float*)wcr, (float*)bb, T, NP,
+                                   ldb, cus, st);
+        }
+        if (urc || hipGetLastError() != hipSuccess)
+            return report_error(AMICA_E_DEVICE, "unmixing GEMM launch failed");
+        a.bbuf = bb;
+    }
     const size_t shm = (size_t)S * NP * rb + (size_t)S * nw * 8 + 2 * (size_t)S * M * 8;
     int rc;
     if (real_bits == 64)
@@ -636,17 +766,33 @@ int ll_call(int real_bits, const void* x, long T, int n, int M, int m, const dou
         rc = adj ? launch_ll<float, true>(m, a, nb, shm, st) : launch_ll<float, false>(m, a, nb, shm, st);
     if (rc) return report_error(AMICA_E_DEVICE, "likelihood kernel launch failed");
     if (adj) {
-        long gch = (T + gchunks - 1) / gchunks;
-        gch = (gch + 31) / 32 * 32;
-        const int gc = (int)((T + gch - 1) / gch);
-        const dim3 grid(gc, (M * NP + 63) / 64, (n + 63) / 64);
-        if (real_bits == 64)
+        int gc = gchunks;
+        if (gx_mfma) {
+            if (launch_gx_f32((const float*)a.g, (const float*)xp, gxpart, T, NP, ldb, gchunks, st))
+                return report_error(AMICA_E_DEVICE, "statistic GEMM launch failed");
+        } else {
+            long gch = (T + gchunks - 1) / gchunks;
+            gch = (gch + 31) / 32 * 32;
+            gc = (int)((T + gch - 1) / gch);
+            const dim3 grid(gc, (ldb + 63) / 64, (n + 63) / 64);
             hipLaunchKernelGGL(adj_gx_kernel<double>, grid, dim3(256), 0, st, (const double*)a.g,
-                               (const double*)x, T, n, M * NP, gch, gxpart);
-        else
-            hipLaunchKernelGGL(adj_gx_kernel<float>, grid, dim3(256), 0, st, (const float*)a.g,
-                               (const float*)x, T, n, M * NP, gch, gxpart);
-        hipLaunchKernelGGL(adj_finish_kernel, dim3(M), dim3(256), (size_t)n * 8, st, a, nb, gxpart, gc, *out);
+                               (const double*)x, T, n, ldb, gch, gxpart);
+        }
+        /* the partial slots summed across the GPU, then one workgroup per
+         * model assembles the gradients from the totals */
+        double *ptot = nullptr, *gtot = nullptr;
+        const size_t pcount = (size_t)ldb * pair_stride(m) + M, gcount = (size_t)ldb * gxs;
+        if (!ws.alloc((void**)&ptot, pcount * 8) || !ws.alloc((void**)&gtot, gcount * 8))
+            return report_error(AMICA_E_NOMEM, "workspace allocation failed");
+        hipLaunchKernelGGL(adj_slot_sum_kernel, dim3((unsigned)std::min<size_t>(4096, (pcount + 255) / 256)),
+                           dim3(256), 0, st, a.part, a.pstride, nb, pcount, ptot);
+        hipLaunchKernelGGL(adj_slot_sum_kernel, dim3((unsigned)std::min<size_t>(4096, (gcount + 255) / 256)),
+                           dim3(256), 0, st, gxpart, gcount, gc, gcount, gtot);
+        AdjArgs at = a;
+        at.part = ptot;
+        at.pstride = pcount;
+        hipLaunchKernelGGL(adj_finish_kernel, dim3(M), dim3(256), (size_t)n * 8, st, at, 1, gtot, 1, gxs,
+                           *out);
         if (hipGetLastError() != hipSuccess) return report_error(AMICA_E_DEVICE, "adjoint launch failed");
     }
     if (noalloc) return 0;     /* asynchronous: the caller synchronises `stream` */

@@ -220,6 +220,11 @@ int launch_prepare(const MstepArgs& ma, hipStream_t st);
 int mstep_configure(int n);
 /* doubles of M-step scratch the inverse uses after the first 8 + M n n */
 size_t inverse_work_doubles(int n, int M);
+/* Ainv_h = A_h^-1 and ldet_inv[h] = -log|det A_h| for M column-major n x n
+ * matrices by the blocked Gauss-Jordan of the M-step; work holds
+ * 8 + M n n + inverse_work_doubles(n, M) doubles. */
+int launch_gj_inverse(const double* A, double* Ainv, double* ldet_inv, int n, int M, double* work,
+                      hipStream_t st);
 
 /* Preprocessing -------------------------------------------------------------- */
 int launch_colmean(const float* x, long T, int n, double* part, int nchunks, double* mean,

@@ -22,6 +22,7 @@
  * LU of W, which can overflow).
  */
 #include <algorithm>
+#include <cstring>
 
 #include "common.h"
 #include "engine_internal.h"
@@ -522,7 +523,7 @@ __global__ __launch_bounds__(256) void wc_kernel(MstepArgs a) {
 __device__ __forceinline__ void set_ldet(const MstepArgs& a, int h, double lad) {
     const double ldet = -lad;          /* log|det W| = -log|det A| */
     a.ldet[h] = ldet;
-    a.lconst[h] = log(a.gm[h]) + ldet + a.ds->ldetS;
+    if (a.lconst) a.lconst[h] = log(a.gm[h]) + ldet + a.ds->ldetS;   /* null: the inverse alone */
 }
 
 /* W = A^-1 by blocked Gauss-Jordan with implicit partial pivoting.
@@ -950,6 +951,25 @@ int mstep_configure(int n) { return (n >= 1 && n <= kInvMax) ? 0 : -1; }
 /* scratch doubles of the inverse after the first 8 + M n n of work */
 size_t inverse_work_doubles(int n, int M) {
     return (size_t)M * n * n + 2 * (size_t)M * n;
+}
+
+int launch_gj_inverse(const double* A, double* Ainv, double* ldet_inv, int n, int M, double* work,
+                      hipStream_t st) {
+    if (n < 1 || n > kInvMax || M < 1) return -1;
+    MstepArgs ma;
+    std::memset(&ma, 0, sizeof(ma));
+    ma.n = n;
+    ma.M = M;
+    ma.A = const_cast<double*>(A);
+    ma.W = Ainv;
+    ma.ldet = ldet_inv;
+    ma.work = work;
+    if (n <= 64) launch_gj<1, 32>(ma, st);
+    else if (n <= 128) launch_gj<2, 16>(ma, st);
+    else if (n <= 256) launch_gj<4, 16>(ma, st);
+    else if (n <= 512) launch_gj<8, 8>(ma, st);
+    else launch_gj<16, 4>(ma, st);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_prepare(const MstepArgs& ma, hipStream_t st) {

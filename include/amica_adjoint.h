@@ -58,9 +58,11 @@ int amica_ll_forward(int real_bits, const void* x, long T, int n, int M, int m,
 /* amica_ll_forward without allocating: no counterpart in the reference;
  * for callers that cannot allocate device memory (code under hipGraph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
- * arguments and result as amica_ll_forward; its workspace is a static
- * device array of 1.6 MB per GPU (held until the process ends), so it also
- * requires M * n * n <= 65536 and rejects larger models with
+ * arguments and result as amica_ll_forward, up to the rounding of b (it
+ * forms b = W x - W c per lane, amica_ll_forward on MFMA into a buffer);
+ * its workspace is a static device array of 1.6 MB per GPU (held until the
+ * process ends), so it also requires M * n * n <= 65536 and
+ * M * ceil(n/64) * 64 <= 512 and rejects larger models with
  * AMICA_E_INVALID.  The call only enqueues work on `stream` and returns:
  * the caller synchronises it.  Calls on one GPU share the workspace, so
  * calls that overlap on one GPU (other streams, other host threads) race. */

@@ -1,10 +1,13 @@
 """amica_ll_forward_noalloc (the allocation-free variant of amica_ll_forward,
-include/amica_adjoint.h) against amica_ll_forward through the C ABI: the
-same kernels on a static workspace, so the per-sample log-likelihoods must
-be bit-identical, on two calls in a row (the second on a smaller problem,
-after the first left its workspace filled), in FP64 and FP32 density math;
-the first call is also held to the FP64 oracle's log-likelihood sum, and
-models beyond the workspace are rejected before any GPU work."""
+include/amica_adjoint.h) against amica_ll_forward through the C ABI, on two
+calls in a row (the second on a smaller problem, after the first left its
+workspace filled), in FP64 and FP32 density math.  The allocation-free
+variant forms b = W x - W c per lane (it has no room for a [T][M NP]
+buffer) where amica_ll_forward reads it from the MFMA unmixing GEMM, so the
+per-sample log-likelihoods agree to the rounding of b: 1e-12 relative in
+FP64, 2e-6 in FP32.  The first call is also held to the FP64 oracle's
+log-likelihood sum, and models beyond the workspace are rejected before any
+GPU work."""
 import numpy as np
 import pytest
 import torch
@@ -63,7 +66,9 @@ def test_noalloc_forward_matches_allocating_forward(bits):
         xw, ldetS, st, vals = _problem(n, T, M, m, 8100 + 17 * k + bits)
         a, b = _both(xw, ldetS, vals, n, M, m, bits)
         assert np.all(np.isfinite(a))
-        np.testing.assert_array_equal(a, b)
+        rel = float(np.max(np.abs(a - b) / np.abs(b)))
+        _say(f"  max per-sample relative difference {rel:.2e}")
+        assert rel <= (1e-12 if bits == 64 else 2e-6)
         if k == 0 and bits == 64:
             ora = po.estep(xw, ldetS, st["A"], st["c"], st["alpha"], st["mu"], st["beta"],
                            st["rho"], st["gm"], M, m)
