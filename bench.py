@@ -29,7 +29,8 @@ Prints one JSON line.  Fields beyond the driver's contract:
                 same initial state: LL error against the FP64 oracle, per
                 model the Amari distance between the GPU's and the
                 oracle's W, the relative errors of alpha, mu, beta, rho,
-                and each run's Amari distance to the true mixing matrices
+                and each run's Amari distance to the true mixing matrices;
+                for a BF16 / FP8 run the same for the FP32 path (fp32_path)
   adjoint       amica_ll_forward / amica_ll_adjoint (FP32 density math) on the
                 whole whitened recording at the state the iterations reached:
                 milliseconds per call
@@ -146,7 +147,8 @@ def accuracy_vs_fp64(w, x, st0, precision, T_acc=60000, iters=50):
     LL (last and max over the run), per model the Amari distance between the
     two final W and the relative errors of alpha, mu, beta, rho; and each
     model's Amari distance from the nearest true mixing matrix (of the ones
-    that occur in those samples), for both runs."""
+    that occur in those samples), for both runs.  A reduced-precision run
+    also reports the FP32 path on the same problem (fp32_path)."""
     import cudamica_amd as ca
     from oracle import pyoracle as po
     n, M, m = w.n, w.M, w.m
@@ -164,36 +166,44 @@ def accuracy_vs_fp64(w, x, st0, precision, T_acc=60000, iters=50):
     cfg = po.default_config(n, T_acc, M, m, maxiter=iters, newt_start_iter=w.newt_start_iter)
     fin, logs = po.run(cfg, xw, ldetS, st0, nthreads=_cores())
     log(f"accuracy: FP64 oracle, {iters} iterations on {T_acc} samples in {time.perf_counter() - t0:.1f} s")
-    with ca.Engine(w.options(T=T_acc, precision=precision, max_iter=iters)) as eng:
-        eng.load_raw(xs)
-        eng.preprocess(w.do_sphere)
-        Sg = eng.sphere()[0].reshape(n, n).T
-        eng.set_state(st0)
-        eng.iterate(iters)
-        eng.sync()
-        glog = eng.log(iters)
-        gst = eng.get_state()
     So = S.reshape(n, n).T
-    rel = np.abs(glog["ll"] - logs["ll"]) / np.abs(logs["ll"])
     nn, nm = n * n, n * m
-    out = {"samples": T_acc, "iterations": iters, "ll_rel_last": float(rel[-1]),
-           "ll_rel_max": float(rel.max()), "ll_rel_max_iteration": int(np.argmax(rel)) + 1,
-           "decisions_equal": bool(np.array_equal(glog["lldec"], logs["lldec"])
-                                   and np.array_equal(glog["newton"], logs["newton"])),
-           "models": []}
-    for h in range(M):
-        Wg = gst["W"][h * nn:(h + 1) * nn].reshape(n, n).T
-        Wo = fin["W"][h * nn:(h + 1) * nn].reshape(n, n).T
-        sl = slice(h * nm, (h + 1) * nm)
-        d = {"amari_W_vs_oracle": float(_amari(Wg, np.linalg.inv(Wo))),
-             "amari_truth_gpu": float(min(_amari(Wg @ Sg, At) for At in A_true)),
-             "amari_truth_oracle": float(min(_amari(Wo @ So, At) for At in A_true))}
-        for k in ("alpha", "mu", "beta", "rho"):
-            d[f"{k}_rel"] = float(np.linalg.norm(gst[k][sl] - fin[k][sl]) / np.linalg.norm(fin[k][sl]))
-        out["models"].append(d)
-    worst = max(mm["amari_W_vs_oracle"] for mm in out["models"])
-    log(f"accuracy after {iters} iterations on {T_acc} samples: LL rel {rel[-1]:.2e} (max {rel.max():.2e}), "
-        f"worst W Amari vs oracle {worst:.2e}")
+
+    def gpu_run(prec):
+        with ca.Engine(w.options(T=T_acc, precision=prec, max_iter=iters)) as eng:
+            eng.load_raw(xs)
+            eng.preprocess(w.do_sphere)
+            Sg = eng.sphere()[0].reshape(n, n).T
+            eng.set_state(st0)
+            eng.iterate(iters)
+            eng.sync()
+            glog = eng.log(iters)
+            gst = eng.get_state()
+        rel = np.abs(glog["ll"] - logs["ll"]) / np.abs(logs["ll"])
+        res = {"ll_rel_last": float(rel[-1]), "ll_rel_max": float(rel.max()),
+               "ll_rel_max_iteration": int(np.argmax(rel)) + 1,
+               "decisions_equal": bool(np.array_equal(glog["lldec"], logs["lldec"])
+                                       and np.array_equal(glog["newton"], logs["newton"])),
+               "models": []}
+        for h in range(M):
+            Wg = gst["W"][h * nn:(h + 1) * nn].reshape(n, n).T
+            Wo = fin["W"][h * nn:(h + 1) * nn].reshape(n, n).T
+            sl = slice(h * nm, (h + 1) * nm)
+            d = {"amari_W_vs_oracle": float(_amari(Wg, np.linalg.inv(Wo))),
+                 "amari_truth_gpu": float(min(_amari(Wg @ Sg, At) for At in A_true)),
+                 "amari_truth_oracle": float(min(_amari(Wo @ So, At) for At in A_true))}
+            for k in ("alpha", "mu", "beta", "rho"):
+                d[f"{k}_rel"] = float(np.linalg.norm(gst[k][sl] - fin[k][sl]) / np.linalg.norm(fin[k][sl]))
+            res["models"].append(d)
+        worst = max(mm["amari_W_vs_oracle"] for mm in res["models"])
+        log(f"accuracy ({PREC_NAME[prec]}) after {iters} iterations on {T_acc} samples: LL rel "
+            f"{rel[-1]:.2e} (max {rel.max():.2e}), worst W Amari vs oracle {worst:.2e}")
+        return res
+
+    out = {"samples": T_acc, "iterations": iters, **gpu_run(precision)}
+    if precision != 32:
+        # the FP32 path on the same problem, against the same oracle run
+        out["fp32_path"] = gpu_run(32)
     return out
 
 
