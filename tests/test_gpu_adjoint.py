@@ -176,8 +176,12 @@ def test_adjoint_equals_reference_statistics(n, T, M, m):
             assert err <= 1e-10, name
 
 
-def test_adjoint_fp32_matches_fp64():
-    n, T, M, m = 64, 20000, 1, 3
+@pytest.mark.parametrize("n,T,M,m", [(64, 20000, 1, 3), (300, 6000, 2, 3), (100, 5000, 5, 2)],
+                         ids=["n64M1", "n300M2", "n100M5"])
+def test_adjoint_fp32_matches_fp64(n, T, M, m):
+    """The FP32 variant (FP32 MFMA GEMMs, FP32 density math) against the FP64
+    one, within one workgroup and through the three-pass split (the last
+    two shapes)."""
     _say(f"[ll adjoint fp32 vs fp64] n={n} T={T} M={M} m={m}")
     xw, ldetS, st, W = _problem(n, T, M, m, 7404)
     dev = torch.device("cuda:0")
