@@ -113,10 +113,10 @@ error read_key(parsedconfigs* pc, const KeySpec& k) {
     return rc;
 }
 
-/* `devices`: a list of ordinals, default device 0. */
+/* `devices`: a list of ordinals (at most 64), default device 0. */
 error read_devices(parsedconfigs* pc) {
     natural list[64];
-    natural count = 0;
+    natural count = 64;                 /* the capacity of list */
     const error rc = getNaturalList(pc, (char*)"devices", list, &count);
     if (rc == ERRORNOPARAM) {
         currentConfig.devcount = 1;

@@ -90,9 +90,11 @@ error getNatural(parsedconfigs* config, char* string, natural* result) {
     return getLong(config, string, (long*)result);
 }
 
+/* *size > 0 on entry: the capacity of result */
 error getIntList(parsedconfigs* config, char* string, int* result, int* size) {
     std::vector<std::string> v;
     if (!find_key(config, string, &v)) return ERRORNOPARAM;
+    if (size && *size > 0 && (long)v.size() > (long)*size) return ERRORINVALIDPARAM;
     int n = 0;
     for (const auto& s : v) result[n++] = atoi(s.c_str());
     if (size) *size = n;
@@ -102,6 +104,7 @@ error getIntList(parsedconfigs* config, char* string, int* result, int* size) {
 error getLongList(parsedconfigs* config, char* string, long* result, long* size) {
     std::vector<std::string> v;
     if (!find_key(config, string, &v)) return ERRORNOPARAM;
+    if (size && *size > 0 && (long)v.size() > *size) return ERRORINVALIDPARAM;
     long n = 0;
     for (const auto& s : v) result[n++] = atol(s.c_str());
     if (size) *size = n;

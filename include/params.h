@@ -5,7 +5,10 @@
  * reference tests a substring first, src/params.c:12-17, which lets e.g.
  * `afile` match inside `datafile` lines); trailing newlines are trimmed
  * instead of dropping the last character (params.c:205); the list
- * accessors always set *size.  Lines starting with '#' are comments.
+ * accessors always set *size, and a *size above 0 on entry is the capacity
+ * of result: a longer list stores nothing and returns ERRORINVALIDPARAM
+ * (the reference writes past the array; callers that pass a zeroed *size
+ * keep its unbounded behaviour).  Lines starting with '#' are comments.
  */
 #ifndef __PARAMS_H__
 #define __PARAMS_H__

@@ -125,6 +125,11 @@ def test_devices_list_and_precision(cfg, tmp_path):
     assert rc == 0 and c.precision == 8
     rc, c = cfg(base + "precision fp4\n", str(tmp_path))
     assert rc == -3
+    rc, c = cfg(base + "precision fp32plain\n", str(tmp_path))
+    assert rc == 0 and c.precision == 33
+    # more device ordinals than the list holds: an error, nothing written past it
+    rc, c = cfg(base + "devices" + " 1" * 70 + "\n", str(tmp_path))
+    assert rc == -3
 
 
 def test_missing_file(cfg):
