@@ -248,7 +248,7 @@ constexpr int kUnmixStages = 4;
 
 /* The workgroup's tiles: tile = first + j stride for j < ntl; column group
  * cgroup (of 128 CT columns). */
-template <int CT, int KS, bool Q8>
+template <int CT, int KS, bool Q8, int TPW = 1>
 __device__ __forceinline__ void unmix_lds_body(const void* __restrict__ xa,
                                                const uint16_t* __restrict__ xlo,
                                                const uint16_t* __restrict__ Whi,
@@ -258,23 +258,30 @@ __device__ __forceinline__ void unmix_lds_body(const void* __restrict__ xa,
                                                const long first, const long stride, const long ntl,
                                                const int cgroup) {
     constexpr int NP = 32 * KS;
+    /* TPW 16-sample tiles per step, each computed by WPT waves: recordings
+     * whose columns fill fewer than 8 waves (one 64-channel model: ldb = 64)
+     * put the other waves on further tiles instead of leaving them idle */
+    constexpr int WPT = 8 / TPW;
+    constexpr int ROWS = 16 * TPW;                 /* sample rows per step */
     constexpr int PARTS = Q8 ? 1 : 2;              /* fp8 q, or bf16 hi and lo */
     constexpr int ROWB = Q8 ? NP : 2 * NP;         /* bytes of one sample row of one part */
     constexpr int RS = ROWB + 16;                  /* LDS row stride */
-    constexpr int SLOT = PARTS * 16 * RS;          /* LDS bytes of one tile */
+    constexpr int SLOT = PARTS * ROWS * RS;        /* LDS bytes of one step */
     constexpr int CPR = ROWB / 16;                 /* 16-byte chunks per row */
-    constexpr int CHUNKS = PARTS * 16 * CPR;       /* chunks per tile */
+    constexpr int CHUNKS = PARTS * ROWS * CPR;     /* chunks per step */
     constexpr int NCH = (CHUNKS + 511) / 512;      /* chunks per lane */
     /* stages in flight: 3 where the W fragments of a split-BF16 NP = 256
-     * wave (2 x 8 pairs) leave no room for a fourth without spilling */
-    constexpr int D = (!Q8 && KS == 8) ? kUnmixStages - 1 : kUnmixStages;
+     * wave (2 x 8 pairs) leave no room for a fourth without spilling, 2
+     * where a step holds several tiles */
+    constexpr int D = TPW > 1 ? 2 : (!Q8 && KS == 8) ? kUnmixStages - 1 : kUnmixStages;
     __shared__ __align__(16) unsigned char lds[2 * SLOT];
     /* per wave: its 16 x 16 CT output tile, rows padded by 4 floats */
     constexpr int OST = 16 * CT + 4;
     __shared__ __align__(16) float ostage[8][16 * OST];
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int c0 = (cgroup * 8 + w) * 16 * CT;
+    const int tu = w / WPT;                        /* this wave's tile of the step */
+    const int c0 = (cgroup * WPT + (w - tu * WPT)) * 16 * CT;
     const bool computes = c0 < ldb;                /* wave-uniform */
     const int kofs = (lane >> 4) * 8;
     i16x8 bhi[CT][KS], blo[CT][KS];
@@ -297,10 +304,10 @@ __device__ __forceinline__ void unmix_lds_body(const void* __restrict__ xa,
             const int c = tid + q * 512;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (c < CHUNKS && j < ntl) {
-                const int part = c / (16 * CPR);
-                const int rem = c - part * 16 * CPR;
+                const int part = c / (ROWS * CPR);
+                const int rem = c - part * ROWS * CPR;
                 const int srow = rem / CPR, cc = rem - srow * CPR;
-                const long t = tile * 16 + srow;
+                const long t = tile * ROWS + srow;
                 if (t < T) {
                     const unsigned char* src =
                         Q8 ? (const unsigned char*)xa + (size_t)t * NP + cc * 16
@@ -316,16 +323,16 @@ __device__ __forceinline__ void unmix_lds_body(const void* __restrict__ xa,
         for (int q = 0; q < NCH; ++q) {
             const int c = tid + q * 512;
             if (c < CHUNKS) {
-                const int part = c / (16 * CPR);
-                const int rem = c - part * 16 * CPR;
+                const int part = c / (ROWS * CPR);
+                const int rem = c - part * ROWS * CPR;
                 const int srow = rem / CPR, cc = rem - srow * CPR;
-                *(uint4*)(lds + slot * SLOT + part * 16 * RS + srow * RS + cc * 16) = r[q];
+                *(uint4*)(lds + slot * SLOT + part * ROWS * RS + srow * RS + cc * 16) = r[q];
             }
         }
     };
     auto compute = [&](int slot, long j) {
         const long tile = first + j * stride;
-        const unsigned char* base = lds + slot * SLOT + (lane & 15) * RS;
+        const unsigned char* base = lds + slot * SLOT + (tu * 16 + (lane & 15)) * RS;
         f32x4 acc[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) acc[c] = (f32x4){0.f, 0.f, 0.f, 0.f};
@@ -336,7 +343,7 @@ __device__ __forceinline__ void unmix_lds_body(const void* __restrict__ xa,
                 ahi = fp8x8_to_bf16x8(*(const uint2*)(base + k * 32 + kofs));
             } else {
                 ahi = *(const i16x8*)(base + (k * 32 + kofs) * 2);
-                alo = *(const i16x8*)(base + 16 * RS + (k * 32 + kofs) * 2);
+                alo = *(const i16x8*)(base + ROWS * RS + (k * 32 + kofs) * 2);
             }
 #pragma unroll
             for (int c = 0; c < CT; ++c) {
@@ -363,7 +370,7 @@ __device__ __forceinline__ void unmix_lds_body(const void* __restrict__ xa,
 #pragma unroll
         for (int i = 0; i < 16 / RPI; ++i) {
             const int row = i * RPI + rl;
-            const long t = tile * 16 + row;
+            const long t = tile * ROWS + tu * 16 + row;
             const f32x4 v = *(const f32x4*)(os + row * OST + cl);
             if (t < T) *(f32x4*)(b + (size_t)t * ldb + c0 + cl) = v;
         }
@@ -387,17 +394,18 @@ __device__ __forceinline__ void unmix_lds_body(const void* __restrict__ xa,
     }
 }
 
-template <int CT, int KS, bool Q8>
+/* nsteps steps of TPW 16-sample tiles: step = blockIdx.x, += gridDim.x */
+template <int CT, int KS, bool Q8, int TPW>
 __global__ __launch_bounds__(512) void unmix_lds_kernel(const void* __restrict__ xa,
                                                          const uint16_t* __restrict__ xlo,
                                                          const uint16_t* __restrict__ Whi,
                                                          const uint16_t* __restrict__ Wlo,
                                                          const float* __restrict__ wc,
                                                          float* __restrict__ b, long T, int ldb,
-                                                         long ntiles) {
+                                                         long nsteps) {
     const long first = blockIdx.x, stride = gridDim.x;
-    const long ntl = first < ntiles ? (ntiles - 1 - first) / stride + 1 : 0;   /* block-uniform */
-    unmix_lds_body<CT, KS, Q8>(xa, xlo, Whi, Wlo, wc, b, T, ldb, first, stride, ntl, (int)blockIdx.y);
+    const long ntl = first < nsteps ? (nsteps - 1 - first) / stride + 1 : 0;   /* block-uniform */
+    unmix_lds_body<CT, KS, Q8, TPW>(xa, xlo, Whi, Wlo, wc, b, T, ldb, first, stride, ntl, (int)blockIdx.y);
 }
 
 /* Many recordings in one launch (same NP, same column groups): workgroup b
@@ -928,27 +936,40 @@ static int launch_unmix_reg(const void* xa, const uint16_t* xlo, const uint16_t*
     int CT, KS;
     unmix_shape(NP, CT, KS);
     if (CT == 0 || ldb % (16 * CT)) return -1;
-    const int groups = (ldb + 128 * CT - 1) / (128 * CT);
-    const long ntiles = (T + 15) / 16;
+    /* tiles per step: the waves one column group leaves free take further
+     * tiles (1, 2, 4 or 8 tiles of 16 samples per step) */
+    const int need = (ldb + 16 * CT - 1) / (16 * CT);          /* waves for all columns */
+    const int tpw = need >= 5 ? 1 : need >= 3 ? 2 : need == 2 ? 4 : 8;
+    const int groups = (ldb + (8 / tpw) * 16 * CT - 1) / ((8 / tpw) * 16 * CT);
+    const long nsteps = ((T + 15) / 16 + tpw - 1) / tpw;
     /* as many workgroups as the CUs hold at once (queried once per kernel) */
-#define AMICA_UNMIX_REG(C, K)                                                                   \
+#define AMICA_UNMIX_REG(C, K, P)                                                                \
     do {                                                                                        \
         static int bpc = 0;                                                                     \
         if (bpc == 0) {                                                                         \
             int nb = 0;                                                                         \
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                  \
-                    &nb, unmix_lds_kernel<C, K, Q8>, 512, 0) != hipSuccess || nb < 1)          \
+                    &nb, unmix_lds_kernel<C, K, Q8, P>, 512, 0) != hipSuccess || nb < 1)       \
                 nb = 1;                                                                         \
             bpc = nb;                                                                           \
         }                                                                                       \
-        const long gx = std::max(1L, std::min<long>(ntiles, (long)cus * bpc / groups));         \
-        hipLaunchKernelGGL((unmix_lds_kernel<C, K, Q8>), dim3((unsigned)gx, groups), dim3(512), \
-                           0, st, xa, xlo, Whi, Wlo, wc, b, T, ldb, ntiles);                   \
+        const long gx = std::max(1L, std::min<long>(nsteps, (long)cus * bpc / groups));         \
+        hipLaunchKernelGGL((unmix_lds_kernel<C, K, Q8, P>), dim3((unsigned)gx, groups),         \
+                           dim3(512), 0, st, xa, xlo, Whi, Wlo, wc, b, T, ldb, nsteps);         \
     } while (0)
-    if (CT == 4 && KS == 2) AMICA_UNMIX_REG(4, 2);
-    else if (CT == 4 && KS == 4) AMICA_UNMIX_REG(4, 4);
-    else if (CT == 2 && KS == 6) AMICA_UNMIX_REG(2, 6);
-    else if (CT == 2 && KS == 8) AMICA_UNMIX_REG(2, 8);
+    if (CT == 4 && KS == 2) {
+        if (tpw == 8) AMICA_UNMIX_REG(4, 2, 8);
+        else if (tpw == 4) AMICA_UNMIX_REG(4, 2, 4);
+        else if (tpw == 2) AMICA_UNMIX_REG(4, 2, 2);
+        else AMICA_UNMIX_REG(4, 2, 1);
+    } else if (CT == 4 && KS == 4) {
+        /* NP = 128: ldb >= 128 needs two waves, so at most 4 tiles per step
+         * (8 would not fit the LDS) */
+        if (tpw >= 4) AMICA_UNMIX_REG(4, 4, 4);
+        else if (tpw == 2) AMICA_UNMIX_REG(4, 4, 2);
+        else AMICA_UNMIX_REG(4, 4, 1);
+    } else if (CT == 2 && KS == 6) AMICA_UNMIX_REG(2, 6, 1);
+    else if (CT == 2 && KS == 8) AMICA_UNMIX_REG(2, 8, 1);
     else return -1;
 #undef AMICA_UNMIX_REG
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1026,12 +1047,15 @@ int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, 
 static int gx_bf16_ct(int NP) { return NP % 128 == 0 ? 8 : 4; }
 
 int gx_bf16_chunks(long T, int NP, int ldb, int cus) {
-    /* at least one workgroup per CU of the budget where T allows, and at
+    /* about 8 waves per CU of the budget where T allows (a workgroup has
+     * min(ldb, 512) / 64 waves: one for a single 64-channel model), and at
      * most 256 MB of partials */
     const long nblk = (T + 31) / 32;
     const int slabs = NP / (16 * gx_bf16_ct(NP)) * ((ldb + 511) / 512);
+    const int wpg = std::min(ldb, 512) / 64;
     const long by_mem = std::max(1L, (256L << 20) / ((long)ldb * NP * 8));
-    return (int)std::max(1L, std::min(std::min(nblk, by_mem), (long)std::max(1, cus / slabs)));
+    const long want = std::max(1L, (long)cus * 8 / ((long)slabs * wpg));
+    return (int)std::max(1L, std::min(std::min(nblk, by_mem), want));
 }
 
 int launch_gx_bf16(const uint16_t* g8, const uint16_t* x8, double* part, long T, int NP, int ldb,
