@@ -402,7 +402,8 @@ int amica_engine_create(const amica_options* opts, amica_engine** out) {
      * streaming one contiguous range of samples. */
     const int waves = e->split ? e->sp_mg * e->sp_cwg : M * CW;
     {
-        const int rc = estep_configure(m, estep_lds_bytes(m, 64 * waves));
+        const int rc = estep_configure(m, estep_lds_bytes(m, 64 * waves, M == 1 && !e->split && !e->bf16 &&
+                                                                           !e->plain));
         if (rc) {
             amica_engine_destroy(e);
             return fail(AMICA_E_DEVICE, "E-step LDS configuration failed");

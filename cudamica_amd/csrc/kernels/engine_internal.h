@@ -112,7 +112,7 @@ int launch_estep_m1_batch(const EstepArgs* args, const BatchItem* items, int nit
 /* Allows the E-step variants for m components `lds_bytes` of dynamic LDS. */
 int estep_configure(int m, size_t lds_bytes);
 /* Dynamic LDS of one E-step workgroup of `threads` threads. */
-size_t estep_lds_bytes(int m, int threads);
+size_t estep_lds_bytes(int m, int threads, bool m1b64 = false);
 /* Resident E-step workgroups per CU (registers and LDS), or -1; m1 for the
  * single-model kernel (M = 1, no per-sample L recorded), split for the
  * split launch's statistics pass. */

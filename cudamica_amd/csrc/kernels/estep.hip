@@ -523,6 +523,15 @@ __device__ __forceinline__ void estep_m1_body(const EstepArgs& a, const long t0,
     const bool act = i < a.n;
     __shared__ double llw[kMaxEstepWaves];
 
+    constexpr int NSP = NS | 1;
+    extern __shared__ float tot2_all[];
+    float* const tot2 = tot2_all + threadIdx.x * NSP;
+    /* FP32 path, m = 3: the FP64 constants live in LDS ([2 j + k][thread]
+     * after the totals, each thread its own slots), read once per sample:
+     * 12 VGPRs fewer, so that 4 waves per SIMD fit */
+    constexpr bool kC64Lds = B64 && MJ == 3;
+    double* const c64 = (double*)((char*)tot2_all + ((size_t)blockDim.x * NSP * sizeof(float) + 7) / 8 * 8);
+
     float sb[MJ], nsbmu[MJ], rm2[MJ], cq2[MJ], rsb[MJ];
     double sb64[B64 ? MJ : 1], nsbmu64[B64 ? MJ : 1];
 #pragma unroll
@@ -538,11 +547,12 @@ __device__ __forceinline__ void estep_m1_body(const EstepArgs& a, const long t0,
             const double* dj = a.cst64 + (size_t)j * a.NP + i;
             sb64[j] = act ? dj[kCst64Sb * ks] : 0.0;
             nsbmu64[j] = act ? dj[kCst64Nsbmu * ks] : 1.0;
+            if constexpr (kC64Lds) {
+                c64[(size_t)(2 * j) * blockDim.x + threadIdx.x] = sb64[j];
+                c64[(size_t)(2 * j + 1) * blockDim.x + threadIdx.x] = nsbmu64[j];
+            }
         }
     }
-    constexpr int NSP = NS | 1;
-    extern __shared__ float tot2_all[];
-    float* const tot2 = tot2_all + threadIdx.x * NSP;
     float acc[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -608,11 +618,17 @@ __device__ __forceinline__ void estep_m1_body(const EstepArgs& a, const long t0,
         }
         ld(t + PF, bq[PF - 1], dq[PF - 1], xq[PF - 1]);
         const float bv = B64 ? (float)dcur : bcur;
+        /* the constants' LDS reads stay in the loop (not hoisted back into
+         * registers): a compiler-only fence, no instruction */
+        if constexpr (kC64Lds) __asm__ volatile("" ::: "memory");
         float q2[MJ], E[MJ], SY[MJ], YM2[MJ], YR[MJ], LA[MJ];
 #pragma unroll
         for (int j = 0; j < MJ; ++j) {
             float y;
-            if constexpr (B64) y = (float)fma(sb64[j], dcur, nsbmu64[j]);
+            if constexpr (kC64Lds)
+                y = (float)fma(c64[(size_t)(2 * j) * blockDim.x + threadIdx.x], dcur,
+                               c64[(size_t)(2 * j + 1) * blockDim.x + threadIdx.x]);
+            else if constexpr (B64) y = (float)fma(sb64[j], dcur, nsbmu64[j]);
             else y = fmaf(sb[j], bv, nsbmu[j]);
             LA[j] = fast_log2(fmaxf(fabsf(y), y_floor<B64>()));
             YM2[j] = fast_exp2(rm2[j] * LA[j]);
@@ -855,13 +871,16 @@ int estep_configure(int MJ, size_t lds_bytes) {
     return r == hipSuccess ? 0 : -1;
 }
 
-size_t estep_lds_bytes(int MJ, int threads) {
-    return (size_t)((kStatsPerJ * MJ + kStatsPerI) | 1) * threads * sizeof(float);
+size_t estep_lds_bytes(int MJ, int threads, bool m1b64) {
+    const size_t tot = (size_t)((kStatsPerJ * MJ + kStatsPerI) | 1) * threads * sizeof(float);
+    /* the single-model FP32-path kernel at m = 3 also keeps its FP64
+     * constants there (2 m doubles per thread after the totals) */
+    return (m1b64 && MJ == 3) ? (tot + 7) / 8 * 8 + (size_t)2 * MJ * threads * sizeof(double) : tot;
 }
 
 int estep_blocks_per_cu(int MJ, bool b64, int threads, bool m1, bool split) {
     int nb = 0;
-    const size_t shm = estep_lds_bytes(MJ, threads);
+    const size_t shm = estep_lds_bytes(MJ, threads, m1 && b64);
     hipError_t r = hipErrorInvalidValue;
 #define AMICA_ESTEP_OCC(J)                                                                        \
     case J:                                                                                       \
@@ -958,9 +977,9 @@ int launch_estep(const EstepArgs& a, int MJ, int nblocks, hipStream_t st) {
      * tile past it) must fit the 31-bit scalar offsets of its buffer loads */
     if ((a.chunk + kEstepRowPad) * (long)a.ldb * 8 >= (1L << 31)) return -1;
     dim3 grid(nblocks), block(64 * a.M * a.CW);
-    const size_t shm = estep_lds_bytes(MJ, (int)block.x);
     const bool b64 = a.b64 != nullptr;
     const bool m1 = a.M == 1 && a.ltall == nullptr;
+    const size_t shm = estep_lds_bytes(MJ, (int)block.x, m1 && b64);
 #define AMICA_ESTEP_CASE(J)                                                                   \
     case J:                                                                                   \
         if (m1) {                                                                             \
