@@ -168,7 +168,7 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
 
     /* L'_h = (log gm + ldet + ldetS) - c0 with c0 = max_h of that constant,
      * so that only differences enter FP32 (in log2 units). */
-    if (threadIdx.x < a.M) {
+    if ((int)threadIdx.x < a.M) {
         double c0 = a.lconst[0];
         for (int hh = 1; hh < a.M; ++hh) c0 = fmax(c0, a.lconst[hh]);
         dc2[threadIdx.x] = (float)((a.lconst[threadIdx.x] - c0) * 1.4426950408889634);
@@ -752,7 +752,7 @@ __global__ __launch_bounds__(256) void estep_post_kernel(EstepArgs a) {
     __shared__ float dc2[AMICA_MAX_M];
     __shared__ double red[4][AMICA_MAX_M + 1];
     const int M = a.M, ncg = a.ncg;
-    if (threadIdx.x < M) {
+    if ((int)threadIdx.x < M) {
         double c0 = a.lconst[0];
         for (int hh = 1; hh < M; ++hh) c0 = fmax(c0, a.lconst[hh]);
         dc2[threadIdx.x] = (float)((a.lconst[threadIdx.x] - c0) * 1.4426950408889634);
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(256) void estep_post_kernel(EstepArgs a) {
         }
     }
     __syncthreads();
-    if (threadIdx.x <= M) {
+    if ((int)threadIdx.x <= M) {
         const double tot = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
         const int NS = kStatsPerJ * a.mj + kStatsPerI;
         a.part[(size_t)blockIdx.x * a.pstride + (size_t)M * NS * a.NP + threadIdx.x] = tot;
