@@ -17,12 +17,14 @@
  *   W = pinv(A), ldet          src/amica.cu:27-127, 835-924
  * The reference computes pinv with a host SVD; for the non-singular A of an
  * EM run pinv(A) = A^-1, computed here by Gauss-Jordan elimination with
- * partial pivoting (in LDS for n <= 128, in an L2-resident copy above), and
+ * partial pivoting (in the registers of one workgroup per model for
+ * n <= 192, blocked by 16-column panels in an L2-resident copy above), and
  * log|det W| = -sum log|pivot| (the reference multiplies the diagonal of the
  * LU of W, which can overflow).
  */
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "common.h"
 #include "engine_internal.h"
@@ -871,6 +873,236 @@ __global__ __launch_bounds__(1024) void gj_gather_kernel(MstepArgs a) {
     }
 }
 
+/* 32-bit maximum over the wave, in every lane: row maxima by DPP (quad
+ * permutes, then the half-row and row mirrors), then the four rows' */
+template <typename I>
+__device__ __forceinline__ I wave_max_bcast(I v) {
+    auto dmax = [](I x, auto ctrl) {
+        const I o = (I)__builtin_amdgcn_mov_dpp((int)x, decltype(ctrl)::value, 0xf, 0xf, false);
+        return o > x ? o : x;
+    };
+    v = dmax(v, std::integral_constant<int, 0xB1>());
+    v = dmax(v, std::integral_constant<int, 0x4E>());
+    v = dmax(v, std::integral_constant<int, 0x141>());
+    v = dmax(v, std::integral_constant<int, 0x140>());
+    const I r0 = (I)__builtin_amdgcn_readlane((int)v, 0), r1 = (I)__builtin_amdgcn_readlane((int)v, 16);
+    const I r2 = (I)__builtin_amdgcn_readlane((int)v, 32), r3 = (I)__builtin_amdgcn_readlane((int)v, 48);
+    const I m01 = r0 > r1 ? r0 : r1, m23 = r2 > r3 ? r2 : r3;
+    return m01 > m23 ? m01 : m23;
+}
+
+/* Pivot row of the column held as col[r] (rows lane + 64 r): the largest
+ * |value| among the unused rows (st < 0; NaN counts as infinite), ties to
+ * the smallest row index -- the rule of gj_col_argmax; wave-uniform.  The
+ * bit patterns of non-negative doubles order like their values, so the
+ * maximum is found on the high words (used rows -1) and, only where several
+ * rows share the largest high word, on their low words. */
+template <int R>
+__device__ __forceinline__ int gj_pivot(const double (&col)[R], const int (&st)[R]) {
+    int hi[R];
+    unsigned lo[R];
+    int mh = -1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double x = fabs(col[r]);
+        if (x != x) x = __builtin_inf();
+        const uint64_t u = __builtin_bit_cast(uint64_t, x);
+        hi[r] = st[r] < 0 ? (int)(u >> 32) : -1;
+        lo[r] = (unsigned)u;
+        mh = max(mh, hi[r]);
+    }
+    mh = wave_max_bcast<int>(mh);
+    unsigned long long cand[R];
+    int nc = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        cand[r] = __ballot(hi[r] == mh);
+        nc += __builtin_popcountll(cand[r]);
+    }
+    if (nc > 1) {                           /* equal high words: compare the low words */
+        unsigned ml = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (hi[r] == mh) ml = max(ml, lo[r]);
+        ml = wave_max_bcast<unsigned>(ml);
+#pragma unroll
+        for (int r = 0; r < R; ++r) cand[r] = __ballot(hi[r] == mh && lo[r] == ml);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (cand[r]) return 64 * r + __builtin_ctzll(cand[r]);
+    return 0;       /* not reached: an unused row holds the maximum */
+}
+
+/* n <= 192: the whole inverse of one model by one workgroup, the matrix in
+ * registers.  Wave w holds columns [w C, w C + C), lane l rows l + 64 r.
+ * A step's only exchange is the pivot column -- its values, pivot row and
+ * 1/d -- which the wave that owns it publishes in LDS (double buffered)
+ * before the step's one barrier; each wave reads the pivot row's entries
+ * of its own columns by readlane.  The owner of the next pivot column
+ * updates that column first, at raised issue priority, and finds its
+ * pivot, so that the next step can start as soon as the other waves have
+ * finished this one.  Same elimination and pivot rule as the blocked
+ * kernels above, except that a pivot row is not divided by its d: it
+ * keeps a_pj (and 1 in column k) with multiplier 0 in its own step, and
+ * in later steps takes the same update as every row, which holds the row
+ * up to the factor 1/d, applied once when W is written -- so a step costs
+ * one FMA per element and no per-column work.  W is written through the
+ * pivot permutation directly and log|det A| summed from the pivots kept in
+ * LDS: one launch in all. */
+template <int R, int C>
+__global__ __launch_bounds__(1024) void gj_small_kernel(MstepArgs a) {
+    const int h = blockIdx.x, n = a.n, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    __shared__ double pcol[2][R * 64];      /* pivot column before its step */
+    __shared__ double pinv[2];              /* 1 / d */
+    __shared__ int prow[2];                 /* pivot row */
+    __shared__ int pivk[R * 64];            /* pivot row of step k */
+    __shared__ double dk[R * 64];           /* |d| of step k */
+    __shared__ double invk[R * 64];         /* 1 / d of step k */
+    const double* A = a.A + (size_t)h * n * n;
+    double v[R][C];
+    int st[R];                              /* step that pivoted row lane + 64 r, -1 before */
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        st[r] = i < n ? -1 : n;             /* padding rows count as used */
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int j = w * C + c;
+            v[r][c] = (i < n && j < n) ? A[i + (size_t)j * n] : 0.0;
+        }
+    }
+    /* pivot of column k, this wave's local column cl, published in buffer b */
+    auto publish = [&](int cl, int k, int b) {
+        double col[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) col[r] = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (c == cl) {                  /* wave-uniform */
+#pragma unroll
+                for (int r = 0; r < R; ++r) col[r] = v[r][c];
+            }
+        const int p = gj_pivot<R>(col, st);
+        double d = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (r == (p >> 6)) d = readlane_f64(col[r], p & 63);
+        /* 1/d: hardware estimate and two Newton steps (within an ulp) */
+        double inv = __builtin_amdgcn_rcp(d);
+        inv = fma(fma(-d, inv, 1.0), inv, inv);
+        inv = fma(fma(-d, inv, 1.0), inv, inv);
+#pragma unroll
+        for (int r = 0; r < R; ++r) pcol[b][lane + 64 * r] = col[r];
+        if (lane == 0) {
+            prow[b] = p;
+            pinv[b] = inv;
+            pivk[k] = p;
+            dk[k] = fabs(d);
+            invk[k] = inv;
+        }
+    };
+    if (w == 0) publish(0, 0, 0);
+    __syncthreads();
+    /* C steps per pass of the outer loop, so that the local columns of a
+     * step and of the next one are compile-time indices: wave ow holds
+     * columns k0 .. k0 + C - 1 */
+    for (int k0 = 0; k0 < n; k0 += C) {
+        const int ow = k0 / C;
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc) {
+            const int k = k0 + cc;
+            if (k >= n) break;
+            const int b = k & 1;
+            const int p = __builtin_amdgcn_readfirstlane(prow[b]);
+            const double inv = pinv[b];
+            const int pl = p & 63, pr = p >> 6;
+            /* the multipliers a_ik / d, 0 in the pivot row, which keeps its
+             * values: row p stands for a_pj / d until W is written */
+            double l[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bool piv_lane = lane + 64 * r == p;
+                if (piv_lane) st[r] = k;
+                l[r] = piv_lane ? 0.0 : pcol[b][lane + 64 * r] * inv;
+            }
+            const bool own = w == ow;           /* column k is this wave's column cc */
+            /* column k + 1: local column cl1 of wave ow or ow + 1 (never
+             * column k of the same wave).  Its owner updates it first, needing
+             * only its pivot-row entry, and publishes its pivot. */
+            const int cl1 = cc + 1 < C ? cc + 1 : 0;
+            const bool next = k + 1 < n && w == (cc + 1 < C ? ow : ow + 1);
+            if (next) {
+                __builtin_amdgcn_s_setprio(3);
+                double pv = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (r == pr) pv = readlane_f64(v[r][cl1], pl);
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r][cl1] = fma(-l[r], pv, v[r][cl1]);
+                publish(cl1, k + 1, b ^ 1);
+                __builtin_amdgcn_s_setprio(0);
+            }
+            /* the pivot row's entries of this wave's other columns (uniform:
+             * scalar registers) */
+            double prv[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) prv[c] = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (r == pr) {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) prv[c] = readlane_f64(v[r][c], pl);
+                }
+            /* a_ij - l_i a_pj, and -l_i in column k, whose pivot-row entry
+             * becomes 1 (standing for 1 / d) */
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (c == cl1 && next) continue;
+                if (c == cc && own) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[r][c] = -l[r];
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (r == pr && lane == pl) v[r][c] = 1.0;
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[r][c] = fma(-l[r], prv[c], v[r][c]);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    /* W[r][m] = a[p_r][pinv[m]]: element a[i][j] (row i scaled by the 1 / d
+     * of its step) goes to W[step of row i][p_j] */
+    double* W = a.W + (size_t)h * n * n;
+    double sc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) sc[r] = lane + 64 * r < n ? invk[st[r]] : 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int j = w * C + c;
+        if (j >= n) continue;               /* wave-uniform */
+        const size_t m = (size_t)pivk[j] * n;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (lane + 64 * r < n) W[m + st[r]] = v[r][c] * sc[r];
+    }
+    if (w == 0) {
+        double l = 0.0;                     /* log|det A| = sum_k log|d_k| */
+        for (int k = lane; k < n; k += 64) l += log(dk[k]);
+        l = wave_sum_f64(l);
+        if (lane == 0) set_ldet(a, h, l);
+    }
+}
+
+template <int R, int C>
+static void launch_gj_small(const MstepArgs& ma, hipStream_t st) {
+    const int nw = (ma.n + C - 1) / C;      /* <= 16 waves */
+    hipLaunchKernelGGL((gj_small_kernel<R, C>), dim3(ma.M), dim3(64 * nw), 0, st, ma);
+}
+
 template <int R, int NB>
 static void launch_gj(const MstepArgs& ma, hipStream_t st) {
     const int n = ma.n;
@@ -886,6 +1118,19 @@ static void launch_gj(const MstepArgs& ma, hipStream_t st) {
         hipLaunchKernelGGL((gj_update_kernel<R, NB>), dim3(n, ma.M), dim3(64), 0, st, ma,
                            (npan - 1) * NB, -1);
     hipLaunchKernelGGL(gj_gather_kernel, dim3(ma.M), dim3(1024), 0, st, ma);
+}
+
+/* W_h = A_h^-1 and log|det W_h| of every model: one workgroup per model,
+ * the matrix in registers, up to 192 channels; the blocked kernels beyond
+ * (registers per lane of a panel: R rows x NB columns <= 64 doubles) */
+static void launch_inverse(const MstepArgs& ma, hipStream_t st) {
+    const int n = ma.n;
+    if (n <= 64) launch_gj_small<1, 4>(ma, st);
+    else if (n <= 128) launch_gj_small<2, 8>(ma, st);
+    else if (n <= 192) launch_gj_small<3, 12>(ma, st);
+    else if (n <= 256) launch_gj<4, 16>(ma, st);
+    else if (n <= 512) launch_gj<8, 8>(ma, st);
+    else launch_gj<16, 4>(ma, st);
 }
 
 }  // namespace
@@ -964,23 +1209,14 @@ int launch_gj_inverse(const double* A, double* Ainv, double* ldet_inv, int n, in
     ma.W = Ainv;
     ma.ldet = ldet_inv;
     ma.work = work;
-    if (n <= 64) launch_gj<1, 32>(ma, st);
-    else if (n <= 128) launch_gj<2, 16>(ma, st);
-    else if (n <= 256) launch_gj<4, 16>(ma, st);
-    else if (n <= 512) launch_gj<8, 8>(ma, st);
-    else launch_gj<16, 4>(ma, st);
+    launch_inverse(ma, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_prepare(const MstepArgs& ma, hipStream_t st) {
-    /* registers per lane of a panel: R rows x NB columns <= 64 doubles */
     const int n = ma.n;
     if (n < 1 || n > kInvMax) return -1;
-    if (n <= 64) launch_gj<1, 32>(ma, st);
-    else if (n <= 128) launch_gj<2, 16>(ma, st);
-    else if (n <= 256) launch_gj<4, 16>(ma, st);
-    else if (n <= 512) launch_gj<8, 8>(ma, st);
-    else launch_gj<16, 4>(ma, st);
+    launch_inverse(ma, st);
     const long total = (long)ma.n * ma.n + (long)ma.n * ma.m;
     const unsigned gx = (unsigned)std::min<long>((total + 255) / 256, 256);
     hipLaunchKernelGGL(prepare_tail_kernel, dim3(gx, ma.M), dim3(256), 0, st, ma);

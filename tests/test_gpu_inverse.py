@@ -1,8 +1,9 @@
 """The FP64 unmixing matrices W_h = A_h^-1 and log|det W_h| that every EM
-iteration derives from the mixing matrices on the GPU (blocked Gauss-Jordan
-with partial pivoting), against numpy's LAPACK inverse and slogdet, for
-sizes on both sides of every panel shape (n <= 64, 128, 256, 512) and for
-matrices that force row exchanges at every step."""
+iteration derives from the mixing matrices on the GPU (Gauss-Jordan with
+partial pivoting: one register-resident workgroup per model up to 192
+channels, blocked by panels beyond), against numpy's LAPACK inverse and
+slogdet, for sizes on both sides of every kernel shape (n <= 64, 128, 192,
+256, 512) and for matrices that force row exchanges at every step."""
 import numpy as np
 import pytest
 
@@ -33,8 +34,10 @@ def _matrices(n, M, kind, rng):
     return out
 
 
-@pytest.mark.parametrize("n,M,kind", [(5, 2, "random"), (64, 2, "random"), (65, 2, "permuted"),
-                                      (128, 4, "random"), (129, 1, "graded"), (200, 2, "permuted"),
+@pytest.mark.parametrize("n,M,kind", [(1, 2, "random"), (5, 2, "random"), (64, 2, "random"),
+                                      (65, 2, "permuted"), (128, 4, "random"), (129, 1, "graded"),
+                                      (150, 2, "graded"), (192, 3, "permuted"), (193, 1, "random"),
+                                      (200, 2, "permuted"),
                                       (256, 2, "random"), (300, 1, "random"), (512, 1, "graded")])
 def test_inverse_and_logdet(built, n, M, kind):
     rng = np.random.default_rng(1000 + n)
