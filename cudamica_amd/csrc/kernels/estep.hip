@@ -538,10 +538,12 @@ __device__ __forceinline__ void estep_m1_body(const EstepArgs& a, const long t0,
     for (int j = 0; j < MJ; ++j) {
         const float* cj = a.cst + (size_t)j * a.NP + i;
         const size_t ks = (size_t)MJ * a.NP;
+        /* a padding channel: y = 1, so q2 = (0, -1e30, ...) and the
+         * component sum is exactly 1 (log 0 to the LL, g = 0) */
         sb[j] = act ? cj[kCstSb * ks] : 0.f;
         nsbmu[j] = act ? cj[kCstNsbmu * ks] : 1.f;
         rm2[j] = act ? cj[kCstRm2 * ks] : 0.f;
-        cq2[j] = act ? cj[kCstCq2 * ks] : 0.f;
+        cq2[j] = act ? cj[kCstCq2 * ks] : (j == 0 ? AMICA_L2E : -1e30f);
         rsb[j] = act ? cj[kCstRsb * ks] : 0.f;
         if constexpr (B64) {
             const double* dj = a.cst64 + (size_t)j * a.NP + i;
@@ -568,9 +570,11 @@ __device__ __forceinline__ void estep_m1_body(const EstepArgs& a, const long t0,
     const size_t bcol = (size_t)i;
     uint4* const g8 = (uint4*)a.g;
 
-    /* a prefetch window of PF samples, rotated one sample per iteration
-     * (3 with FP64 b, which keeps the kernel within 128 VGPRs) */
-    constexpr int PF = B64 ? 3 : 4;
+    /* a ring of PF prefetched samples; the loop runs whole packed
+     * 8-sample groups with the sample loop unrolled, so that ring slots and
+     * packed g words are fixed registers (PF divides 8) */
+    constexpr int PF = 4;
+    static_assert(G % PF == 0, "ring slots must repeat every group");
     float bq[PF], xq[PF];
     double dq[PF];
     /* Unconditional loads through buffer resources based at row t0 (scalar
@@ -600,102 +604,102 @@ __device__ __forceinline__ void estep_m1_body(const EstepArgs& a, const long t0,
         dq[p] = 0.0;
         ld(t0 + p, bq[p], dq[p], xq[p]);
     }
-    /* packed bf16 g: word (k / 2) mod 4 of the 8-sample group */
-    uint32_t gw0 = 0, gw1 = 0, gw2 = 0, gw3 = 0;
-    float gprev = 0.f;
     const long tend = t0 + (t1 - t0 + G - 1) / G * G;   /* whole 8-sample groups */
+    int grp = 0;
 #pragma unroll 1
-    for (long t = t0; t < tend; ++t) {
-        const bool valid = t < t1;
-        const float bcur = B64 ? 0.f : bq[0];
-        const double dcur = dq[0];
-        const float xcur = xq[0];
+    for (long tg = t0; tg < tend; tg += G, ++grp) {
+        uint32_t gw[G / 2];                           /* packed bf16 g, 2 samples a word */
+        float gprev = 0.f;
 #pragma unroll
-        for (int p = 0; p + 1 < PF; ++p) {
-            bq[p] = bq[p + 1];
-            dq[p] = dq[p + 1];
-            xq[p] = xq[p + 1];
-        }
-        ld(t + PF, bq[PF - 1], dq[PF - 1], xq[PF - 1]);
-        const float bv = B64 ? (float)dcur : bcur;
-        /* the constants' LDS reads stay in the loop (not hoisted back into
-         * registers): a compiler-only fence, no instruction */
-        if constexpr (kC64Lds) __asm__ volatile("" ::: "memory");
-        float q2[MJ], E[MJ], SY[MJ], YM2[MJ], YR[MJ], LA[MJ];
-#pragma unroll
-        for (int j = 0; j < MJ; ++j) {
-            float y;
-            if constexpr (kC64Lds)
-                y = (float)fma(c64[(size_t)(2 * j) * blockDim.x + threadIdx.x], dcur,
-                               c64[(size_t)(2 * j + 1) * blockDim.x + threadIdx.x]);
-            else if constexpr (B64) y = (float)fma(sb64[j], dcur, nsbmu64[j]);
-            else y = fmaf(sb[j], bv, nsbmu[j]);
-            LA[j] = fast_log2(fmaxf(fabsf(y), y_floor<B64>()));
-            YM2[j] = fast_exp2(rm2[j] * LA[j]);
-            SY[j] = YM2[j] * y;
-            YR[j] = SY[j] * y;
-            q2[j] = fmaf(YR[j], -AMICA_L2E, cq2[j]);
-        }
-        float se = 0.f, off = 0.f;
-#pragma unroll
-        for (int j = 0; j < MJ; ++j) {
-            E[j] = fast_exp2(q2[j]);
-            se += E[j];
-        }
-        if (__builtin_expect(__any(se < 0x1p-100f), 0)) {
-            off = q2[0];
-#pragma unroll
-            for (int j = 1; j < MJ; ++j) off = fmaxf(off, q2[j]);
-            se = 0.f;
+        for (int u = 0; u < G; ++u) {
+            const long t = tg + u;
+            const bool valid = t < t1;                /* wave-uniform */
+            const int sl = u % PF;
+            const float bcur = B64 ? 0.f : bq[sl];
+            const double dcur = dq[sl];
+            const float xcur = xq[sl];
+            ld(t + PF, bq[sl], dq[sl], xq[sl]);
+            const float bv = B64 ? (float)dcur : bcur;
+            /* the constants' LDS reads stay in the loop (not hoisted back into
+             * registers): a compiler-only fence, no instruction */
+            if constexpr (kC64Lds) __asm__ volatile("" ::: "memory");
+            float q2[MJ], E[MJ], SY[MJ], YM2[MJ], YR[MJ], LA[MJ];
 #pragma unroll
             for (int j = 0; j < MJ; ++j) {
-                E[j] = fast_exp2(q2[j] - off);
+                float y;
+                if constexpr (kC64Lds)
+                    y = (float)fma(c64[(size_t)(2 * j) * blockDim.x + threadIdx.x], dcur,
+                                   c64[(size_t)(2 * j + 1) * blockDim.x + threadIdx.x]);
+                else if constexpr (B64) y = (float)fma(sb64[j], dcur, nsbmu64[j]);
+                else y = fmaf(sb[j], bv, nsbmu[j]);
+                LA[j] = fast_log2(fmaxf(fabsf(y), y_floor<B64>()));
+                YM2[j] = fast_exp2(rm2[j] * LA[j]);
+                SY[j] = YM2[j] * y;
+                YR[j] = SY[j] * y;
+                q2[j] = fmaf(YR[j], -AMICA_L2E, cq2[j]);
+            }
+            float se = 0.f, off = 0.f;
+#pragma unroll
+            for (int j = 0; j < MJ; ++j) {
+                E[j] = fast_exp2(q2[j]);
                 se += E[j];
             }
-        }
-        const bool use = act && valid;
-        const float rs = use ? fast_rcp(se) : 0.f;
-        qacc += use ? off + fast_log2(se) : 0.f;
-        float g = 0.f;
+            if (__builtin_expect(__any(se < 0x1p-100f), 0)) {
+                off = q2[0];
 #pragma unroll
-        for (int j = 0; j < MJ; ++j) {
-            const float u = E[j] * rs;
-            const float uf = u * SY[j];
-            g = fmaf(uf, rsb[j], g);
-            float* aj = acc + j * NSJ;
-            aj[0] += u;
-            aj[1] += uf;
-            aj[2] = fmaf(uf, SY[j], aj[2]);
-            aj[3] = fmaf(u, YM2[j], aj[3]);
-            const float uyr = u * YR[j];
-            aj[4] += uyr;
-            aj[5] = fmaf(uyr, YR[j], aj[5]);
-            aj[6] = fmaf(uyr, LA[j], aj[6]);
-        }
-        acc[NSJ * MJ + 0] = fmaf(bv, bv, acc[NSJ * MJ + 0]);
-        acc[NSJ * MJ + 1] += xcur;
-        acc[NSJ * MJ + 2] += g;
-        const int k = (int)(t - t0);                  /* t0 is a multiple of 8 */
-        if (a.g_bf16) {
-            if (k & 1) {
-                const uint32_t wd = pack_bf16x2(gprev, g);
-                const int q = (k >> 1) & 3;            /* wave-uniform */
-                if (q == 0) gw0 = wd;
-                else if (q == 1) gw1 = wd;
-                else if (q == 2) gw2 = wd;
-                else gw3 = wd;
-                if ((k & 7) == 7) g8[(size_t)((t - 7) >> 3) * a.ldb + bcol] = make_uint4(gw0, gw1, gw2, gw3);
+                for (int j = 1; j < MJ; ++j) off = fmaxf(off, q2[j]);
+                se = 0.f;
+#pragma unroll
+                for (int j = 0; j < MJ; ++j) {
+                    E[j] = fast_exp2(q2[j] - off);
+                    se += E[j];
+                }
             }
-            gprev = g;
-        } else if (valid) {
-            ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
+            /* samples past t1 (the last range's last group) count nothing */
+            const float rs = valid ? fast_rcp(se) : 0.f;
+            qacc += valid ? off + fast_log2(se) : 0.f;
+            float g = 0.f;
+#pragma unroll
+            for (int j = 0; j < MJ; ++j) {
+                const float u_ = E[j] * rs;
+                const float uf = u_ * SY[j];
+                g = fmaf(uf, rsb[j], g);
+                float* aj = acc + j * NSJ;
+                aj[0] += u_;
+                aj[1] += uf;
+                aj[2] = fmaf(uf, SY[j], aj[2]);
+                aj[3] = fmaf(u_, YM2[j], aj[3]);
+                const float uyr = u_ * YR[j];
+                aj[4] += uyr;
+                aj[5] = fmaf(uyr, YR[j], aj[5]);
+                aj[6] = fmaf(uyr, LA[j], aj[6]);
+            }
+            acc[NSJ * MJ + 0] = fmaf(bv, bv, acc[NSJ * MJ + 0]);
+            acc[NSJ * MJ + 1] += xcur;
+            acc[NSJ * MJ + 2] += g;
+            if (a.g_bf16) {
+                if (u & 1) gw[u >> 1] = pack_bf16x2(gprev, g);
+                gprev = g;
+            } else if (valid) {
+                ((float*)a.g)[(size_t)t * a.ldb + bcol] = g;
+            }
+            /* one sample at a time: the sums are complete here (empty asm
+             * with them as operands, no instruction) and the scheduler may
+             * not interleave the unrolled samples -- deferring the sums would
+             * keep every sample's values live, beyond 128 VGPRs */
+#pragma unroll
+            for (int s = 0; s < NS; ++s) __asm__ volatile("" : "+v"(acc[s]));
+            __asm__ volatile("" : "+v"(qacc));
+            __builtin_amdgcn_sched_barrier(0);
         }
-        if ((k & 31) == 31) {
+        /* t0 is a multiple of 8: the group's packed word */
+        if (a.g_bf16) g8[(size_t)((tg) >> 3) * a.ldb + bcol] = make_uint4(gw[0], gw[1], gw[2], gw[3]);
+        if ((grp & 3) == 3) {                         /* 32 samples */
 #pragma unroll
             for (int s = 0; s < NS; ++s) { tot2[s] += acc[s]; acc[s] = 0.f; }
             qtot += (double)qacc;
             qacc = 0.f;
-            if ((k & 1023) == 1023) {
+            if ((grp & 127) == 127) {                 /* 1024 samples */
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
                     l3[s] = l3_started ? l3[s] + (double)tot2[s] : (double)tot2[s];
