@@ -751,17 +751,35 @@ __global__ __launch_bounds__(512) void gx_f32_slab_kernel(const float* __restric
  * block's fragments are loaded while the current block's MFMAs run.  Every
  * g and x element is read from HBM once per slab; FP32 accumulation inside
  * the MFMA (32 products per rounding) over the chunk, FP64 partial per
- * chunk. */
-/* 32-sample blocks [b0, b1) of one recording into its partial slot `slot` */
+ * chunk.  Where g has fewer than 512 columns (ldb/64 = rw row waves), a
+ * workgroup of 8 waves runs S = 8 / rw sub-ranges of its chunk side by
+ * side and adds their FP32 accumulators through LDS in a fixed order
+ * (S - 1 first, 0 last) before the one FP64 partial is written: as many
+ * waves in flight and per-wave accumulation lengths as with S times the
+ * chunks, but S times fewer partials to write and reduce. */
+/* 32-sample blocks [b0, b1) of one recording into its partial slot `slot`;
+ * S = (waves of the workgroup) / rw sub-ranges, red: dynamic LDS of
+ * rw 64 x 16 CT floats when S > 1 */
 template <int CT>
 __device__ __forceinline__ void gx_bf16_packed_body(const uint16_t* __restrict__ g8,
                                                     const uint16_t* __restrict__ x8,
                                                     double* __restrict__ part, const int NP,
-                                                    const int ldb, const long b0, const long b1,
+                                                    const int ldb, long b0, long b1,
                                                     const int slot) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = blockIdx.z * 512 + w * 64;      /* grid.z: groups of 512 g columns */
-    if (r0 >= ldb) return;                          /* no barriers below */
+    const int rw = min(ldb - (int)blockIdx.z * 512, 512) / 64;   /* row waves */
+    /* sample sub-ranges: only below 512 columns, where the launch gives 8
+     * waves for fewer row waves and the LDS tile with them */
+    const int S = ldb < 512 ? max(1, (int)(blockDim.x >> 6) / rw) : 1;
+    const int sr = w / rw, wr = w - sr * rw;
+    const int r0 = blockIdx.z * 512 + wr * 64;     /* grid.z: groups of 512 g columns */
+    if (S == 1 && r0 >= ldb) return;                /* no barriers below when S = 1 */
+    {
+        const long sub = (b1 - b0 + S - 1) / S;     /* this wave's sub-range */
+        const long s0 = min(b1, b0 + sr * sub);
+        b1 = min(b1, s0 + sub);
+        b0 = s0;
+    }
     const int k0 = blockIdx.y * 16 * CT;
     f32x4 acc[4][CT];
 #pragma unroll
@@ -794,6 +812,35 @@ __device__ __forceinline__ void gx_bf16_packed_body(const uint16_t* __restrict__
                 acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, Ac[rt]),
                                                                       __builtin_bit_cast(bf16x8, Bc[ct]),
                                                                       acc[rt][ct], 0, 0, 0);
+    }
+    if (S > 1) {
+        /* sub-ranges S-1 .. 1 into the LDS tile of their row wave, in order;
+         * sub-range 0 adds the total to its own and writes the partial */
+        extern __shared__ float red[];
+        constexpr int RS = 16 * CT + 4;             /* padded row of the tile */
+        float* const tile = red + (size_t)wr * 64 * RS;
+        for (int q = S - 1; q >= 1; --q) {
+            if (sr == q) {
+#pragma unroll
+                for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            float& d = tile[(rt * 16 + (lane >> 4) * 4 + r) * RS + ct * 16 + (lane & 15)];
+                            d = q == S - 1 ? acc[rt][ct][r] : d + acc[rt][ct][r];
+                        }
+            }
+            __syncthreads();
+        }
+        if (sr != 0) return;
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc[rt][ct][r] += tile[(rt * 16 + (lane >> 4) * 4 + r) * RS + ct * 16 + (lane & 15)];
     }
     double* out = part + (size_t)slot * ldb * NP;
 #pragma unroll
@@ -1056,13 +1103,23 @@ int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, 
 
 static int gx_bf16_ct(int NP) { return NP % 128 == 0 ? 8 : 4; }
 
+/* workgroups of 8 waves that split their chunk's samples (g narrower than
+ * 512 columns, the LDS tile within 64 KB), or of min(ldb, 512) / 64 waves */
+static size_t gx_bf16_split_lds(int NP, int ldb) {
+    return (size_t)(ldb / 64) * 64 * (16 * gx_bf16_ct(NP) + 4) * sizeof(float);
+}
+
+static bool gx_bf16_split(int NP, int ldb) {
+    return ldb < 512 && 8 % (ldb / 64) == 0 && gx_bf16_split_lds(NP, ldb) <= 65536;
+}
+
 int gx_bf16_chunks(long T, int NP, int ldb, int cus) {
     /* about 8 waves per CU of the budget where T allows (a workgroup has
-     * min(ldb, 512) / 64 waves: one for a single 64-channel model), and at
-     * most 256 MB of partials */
+     * 8 waves, or min(ldb, 512) / 64 where they do not split the samples),
+     * and at most 256 MB of partials */
     const long nblk = (T + 31) / 32;
     const int slabs = NP / (16 * gx_bf16_ct(NP)) * ((ldb + 511) / 512);
-    const int wpg = std::min(ldb, 512) / 64;
+    const int wpg = gx_bf16_split(NP, ldb) ? 8 : std::min(ldb, 512) / 64;
     const long by_mem = std::max(1L, (256L << 20) / ((long)ldb * NP * 8));
     const long want = std::max(1L, (long)cus * 8 / ((long)slabs * wpg));
     return (int)std::max(1L, std::min(std::min(nblk, by_mem), want));
@@ -1074,12 +1131,15 @@ int launch_gx_bf16(const uint16_t* g8, const uint16_t* x8, double* part, long T,
     const int CT = gx_bf16_ct(NP);
     const long nblk = (T + 31) / 32;
     const long cblk = (nblk + nchunks - 1) / nchunks;
-    /* up to 8 waves of 64 g columns per workgroup, grid.z over the rest */
-    dim3 grid(nchunks, NP / (16 * CT), (ldb + 511) / 512), block(std::min(ldb, 512));
+    /* up to 8 waves of 64 g columns per workgroup, grid.z over the rest;
+     * narrower g: 8 waves over sub-ranges of the chunk */
+    const bool split = gx_bf16_split(NP, ldb);
+    dim3 grid(nchunks, NP / (16 * CT), (ldb + 511) / 512), block(split ? 512 : std::min(ldb, 512));
+    const size_t shm = split ? gx_bf16_split_lds(NP, ldb) : 0;
     if (CT == 8)
-        hipLaunchKernelGGL(gx_bf16_packed_kernel<8>, grid, block, 0, st, g8, x8, part, nblk, NP, ldb, cblk);
+        hipLaunchKernelGGL(gx_bf16_packed_kernel<8>, grid, block, shm, st, g8, x8, part, nblk, NP, ldb, cblk);
     else
-        hipLaunchKernelGGL(gx_bf16_packed_kernel<4>, grid, block, 0, st, g8, x8, part, nblk, NP, ldb, cblk);
+        hipLaunchKernelGGL(gx_bf16_packed_kernel<4>, grid, block, shm, st, g8, x8, part, nblk, NP, ldb, cblk);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1087,11 +1147,15 @@ int launch_gx_bf16_batch(const GxArgs* args, const BatchItem* items, int nitems,
                          hipStream_t st) {
     if (NP % 64 || ldb % 64 || nitems < 1) return -1;
     const int CT = gx_bf16_ct(NP);
-    const dim3 grid(nitems, NP / (16 * CT), (ldb + 511) / 512), block(std::min(ldb, 512));
+    /* the sub-range split of the single-recording launch, so that an item
+     * covering a recording's own chunk sums exactly as that launch does */
+    const bool split = gx_bf16_split(NP, ldb);
+    const dim3 grid(nitems, NP / (16 * CT), (ldb + 511) / 512), block(split ? 512 : std::min(ldb, 512));
+    const size_t shm = split ? gx_bf16_split_lds(NP, ldb) : 0;
     if (CT == 8)
-        hipLaunchKernelGGL(gx_bf16_batch_kernel<8>, grid, block, 0, st, args, items, NP, ldb);
+        hipLaunchKernelGGL(gx_bf16_batch_kernel<8>, grid, block, shm, st, args, items, NP, ldb);
     else
-        hipLaunchKernelGGL(gx_bf16_batch_kernel<4>, grid, block, 0, st, args, items, NP, ldb);
+        hipLaunchKernelGGL(gx_bf16_batch_kernel<4>, grid, block, shm, st, args, items, NP, ldb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
