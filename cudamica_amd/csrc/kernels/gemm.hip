@@ -677,18 +677,32 @@ __global__ __launch_bounds__(256) void unmix_f32_kernel(const float* __restrict_
  * element is read once per slab, x once per workgroup (the other waves hit
  * the caches), each lane's operands of one sample in 16-byte loads (164
  * VGPRs at CT = 8, 3 waves per SIMD: 3.10 -> 2.94 ms at 128 ch x 2M x 4
- * models against 4-byte loads).  FP64 partial per chunk. */
+ * models against 4-byte loads).  FP64 partial per chunk.  Where g has
+ * fewer than 512 columns (rw = ldb/64 row waves), a workgroup of 8 waves
+ * runs S = 8 / rw sub-ranges of its chunk side by side (each as long as a
+ * chunk of the one-wave-per-row layout) and adds their totals in FP64
+ * through LDS in a fixed order before the one partial is written. */
 template <int CT>
 __global__ __launch_bounds__(512) void gx_f32_slab_kernel(const float* __restrict__ g,
                                                           const float* __restrict__ x,
                                                           double* __restrict__ part, long T,
                                                           int NP, int ldb, long chunk) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = blockIdx.z * 512 + w * 64;      /* grid.z: groups of 512 g columns */
-    if (r0 >= ldb) return;                          /* no barriers below */
+    const int rw = min(ldb - (int)blockIdx.z * 512, 512) / 64;   /* row waves */
+    /* sample sub-ranges: only below 512 columns, where the launch gives 8
+     * waves for fewer row waves and the LDS tile with them */
+    const int S = ldb < 512 ? max(1, (int)(blockDim.x >> 6) / rw) : 1;
+    const int sr = w / rw, wr = w - sr * rw;
+    const int r0 = blockIdx.z * 512 + wr * 64;     /* grid.z: groups of 512 g columns */
+    if (S == 1 && r0 >= ldb) return;                /* no barriers below when S = 1 */
     const int k0 = blockIdx.y * 16 * CT;
-    const long ts = (long)blockIdx.x * chunk;
-    const long te = min(T, ts + chunk);
+    long ts = (long)blockIdx.x * chunk;
+    long te = min(T, ts + chunk);
+    {
+        const long sub = ((chunk + S - 1) / S + 3) / 4 * 4;      /* whole 4-sample steps */
+        ts = min(te, ts + sr * sub);
+        te = min(te, ts + sub);
+    }
     f32x4 acc[4][CT];
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
@@ -727,6 +741,44 @@ __global__ __launch_bounds__(512) void gx_f32_slab_kernel(const float* __restric
     /* register r of MFMA (rt, ct) in lane l: row r0 + 4 (4 (l / 16) + r) + rt,
      * column k0 + CT (l % 16) + ct: a lane writes CT consecutive columns */
     double* out = part + (size_t)blockIdx.x * ldb * NP;
+    if (S == 1) {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + 16 * (lane >> 4) + 4 * r + rt;
+                double* o = out + (size_t)row * NP + k0 + CT * (lane & 15);
+#pragma unroll
+                for (int ct = 0; ct < CT; ct += 2)
+                    *(double2*)(o + ct) = make_double2((double)acc[rt][ct][r], (double)acc[rt][ct + 1][r]);
+            }
+        return;
+    }
+    /* sub-ranges S-1 .. 0 add their totals in FP64 into the LDS tile of
+     * their row wave (register layout: lane-consecutive doubles), in that
+     * order; sub-range 0 then writes the partial from the tile */
+    extern __shared__ double red64[];
+    double* const tile = red64 + (size_t)wr * 16 * CT * 64;
+    for (int q = S - 1; q >= 0; --q) {
+        if (sr == q) {
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    /* converted here, in this wave's turn (not hoisted out
+                     * of the q loop: 16 CT doubles beside the accumulators
+                     * would not fit the registers) */
+                    __asm__ volatile("" : "+v"(acc[rt][ct]));
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        double& d = tile[((rt * CT + ct) * 4 + r) * 64 + lane];
+                        d = q == S - 1 ? (double)acc[rt][ct][r] : d + (double)acc[rt][ct][r];
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    if (sr != 0) return;
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
@@ -735,7 +787,8 @@ __global__ __launch_bounds__(512) void gx_f32_slab_kernel(const float* __restric
             double* o = out + (size_t)row * NP + k0 + CT * (lane & 15);
 #pragma unroll
             for (int ct = 0; ct < CT; ct += 2)
-                *(double2*)(o + ct) = make_double2((double)acc[rt][ct][r], (double)acc[rt][ct + 1][r]);
+                *(double2*)(o + ct) = make_double2(tile[((rt * CT + ct) * 4 + r) * 64 + lane],
+                                                   tile[((rt * CT + ct + 1) * 4 + r) * 64 + lane]);
         }
 }
 
@@ -1078,10 +1131,22 @@ static long gx_chunk(long T, int nchunks, int mult) {
 
 static int gx_f32_ct(int NP) { return NP % 128 == 0 ? 8 : 4; }
 
+/* workgroups of 8 waves that split their chunk's samples (g narrower than
+ * 512 columns, the FP64 LDS tile within 64 KB) */
+static size_t gx_f32_split_lds(int NP, int ldb) {
+    return (size_t)(ldb / 64) * 16 * gx_f32_ct(NP) * 64 * sizeof(double);
+}
+
+static bool gx_f32_split(int NP, int ldb) {
+    return ldb < 512 && 8 % (ldb / 64) == 0 && gx_f32_split_lds(NP, ldb) <= 65536;
+}
+
 int gx_f32_chunks(long T, int NP, int ldb, int cus) {
-    /* chunks of >= 512 samples, at most 256 MB of partials, and at least
-     * one per CU of the budget where T allows */
-    const long by_len = (T + 511) / 512;
+    /* chunks of >= 512 samples per wave (S of them where the waves split a
+     * chunk), at most 256 MB of partials, and at least one per CU of the
+     * budget where T allows */
+    const long per = gx_f32_split(NP, ldb) ? 512L * (8 / (ldb / 64)) : 512L;
+    const long by_len = (T + per - 1) / per;
     const long by_mem = std::max(1L, (256L << 20) / ((long)ldb * NP * 8));
     const int slabs = NP / (16 * gx_f32_ct(NP));
     const long want = std::max(1L, (long)cus / slabs);
@@ -1094,10 +1159,13 @@ int launch_gx_f32(const float* g, const float* x, double* part, long T, int NP, 
     const int CT = gx_f32_ct(NP);
     dim3 grid(nchunks, NP / (16 * CT), (ldb + 511) / 512), block(std::min(ldb, 512));
     const long chunk = gx_chunk(T, nchunks, 4);
+    const bool split = gx_f32_split(NP, ldb);
+    if (split) block = dim3(512);
+    const size_t shm = split ? gx_f32_split_lds(NP, ldb) : 0;
     if (CT == 8)
-        hipLaunchKernelGGL(gx_f32_slab_kernel<8>, grid, block, 0, st, g, x, part, T, NP, ldb, chunk);
+        hipLaunchKernelGGL(gx_f32_slab_kernel<8>, grid, block, shm, st, g, x, part, T, NP, ldb, chunk);
     else
-        hipLaunchKernelGGL(gx_f32_slab_kernel<4>, grid, block, 0, st, g, x, part, T, NP, ldb, chunk);
+        hipLaunchKernelGGL(gx_f32_slab_kernel<4>, grid, block, shm, st, g, x, part, T, NP, ldb, chunk);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
