@@ -114,6 +114,28 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2){a, b}, bf2));
 }
 
+/* The FP64 level of a lane's NS statistic sums: l3[s] = l3[s] + add(s)
+ * (add(s) the first time), or 0 where keep is false.  The loads go out B
+ * at a time before their adds, so a flush costs ceil(NS / B) memory round
+ * trips instead of one per sum, for 2 B transient registers. */
+template <int NS, int B, typename F>
+__device__ __forceinline__ void l3_accumulate(double* l3, bool started, bool keep, F add) {
+#pragma unroll
+    for (int s0 = 0; s0 < NS; s0 += B) {
+        double cur[B];
+        if (started) {                  /* wave-uniform */
+#pragma unroll
+            for (int k = 0; k < B; ++k) cur[k] = s0 + k < NS ? l3[s0 + k] : 0.0;
+        } else {
+#pragma unroll
+            for (int k = 0; k < B; ++k) cur[k] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (s0 + k < NS) l3[s0 + k] = keep ? cur[k] + add(s0 + k) : 0.0;
+    }
+}
+
 /* MODE selects the launch the kernel serves:
  *   kEstepFused  all M models x CW channel blocks in one workgroup (M CW <= 8):
  *                the cross-channel and cross-model sums stay on chip;
@@ -129,6 +151,10 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
      * values stay within 128 VGPRs (two workgroups per CU) */
     constexpr int kS = MJ >= 4 ? 1 : 2;
     constexpr int kFlush = 32 / kS;     /* tiles summed in FP32 registers (32 samples) */
+    /* FP64 flushes with batched loads, except in the split launch's
+     * statistics pass, which they would push past 128 VGPRs on the FP32
+     * path */
+    constexpr bool kL3Batched = MODE != kEstepStats;
     constexpr int NSJ = kStatsPerJ;
     constexpr int NS = NSJ * MJ + kStatsPerI;
     constexpr bool kSplit = MODE != kEstepFused;
@@ -470,11 +496,18 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
                 }
                 if (++flushes == kFlush2) {
                     flushes = 0;
+                    if constexpr (kL3Batched) {
+                        if (wreal)
+                            l3_accumulate<NS, 8>(l3, l3_started, true, [&](int s) { return (double)tot2[s]; });
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        const double v = (double)tot2[s];
-                        if (wreal) l3[s] = l3_started ? l3[s] + v : v;
-                        tot2[s] = 0.f;
+                        for (int s = 0; s < NS; ++s) tot2[s] = 0.f;
+                    } else {
+#pragma unroll
+                        for (int s = 0; s < NS; ++s) {
+                            const double v = (double)tot2[s];
+                            if (wreal) l3[s] = l3_started ? l3[s] + v : v;
+                            tot2[s] = 0.f;
+                        }
                     }
                     l3_started = true;
                 }
@@ -484,11 +517,16 @@ __global__ __launch_bounds__(512) void estep_kernel(EstepArgs a) {
             if (a.g_bf16 && wreal) g8[(size_t)(tg >> 3) * a.ldb + bcol] = make_uint4(gw[0], gw[1], gw[2], gw[3]);
     }
     if constexpr (MODE == kEstepLsum) return;
+    if constexpr (kL3Batched) {
+        if (wreal)
+            l3_accumulate<NS, 8>(l3, l3_started, act, [&](int s) { return (double)tot2[s] + (double)acc[s]; });
+    } else {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const double v = (double)tot2[s] + (double)acc[s];
-        const double r = l3_started ? l3[s] + v : v;
-        if (wreal) l3[s] = act ? r : 0.0;
+        for (int s = 0; s < NS; ++s) {
+            const double v = (double)tot2[s] + (double)acc[s];
+            const double r = l3_started ? l3[s] + v : v;
+            if (wreal) l3[s] = act ? r : 0.0;
+        }
     }
     if constexpr (MODE == kEstepStats) return;   /* vsum and the LL sum: estep_post_kernel */
     /* per-model vsum (lanes hh*kS + s of wave 0) and the LL sum (lanes s) */
@@ -700,21 +738,14 @@ __device__ __forceinline__ void estep_m1_body(const EstepArgs& a, const long t0,
             qtot += (double)qacc;
             qacc = 0.f;
             if ((grp & 127) == 127) {                 /* 1024 samples */
+                l3_accumulate<NS, 8>(l3, l3_started, true, [&](int s) { return (double)tot2[s]; });
 #pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    l3[s] = l3_started ? l3[s] + (double)tot2[s] : (double)tot2[s];
-                    tot2[s] = 0.f;
-                }
+                for (int s = 0; s < NS; ++s) tot2[s] = 0.f;
                 l3_started = true;
             }
         }
     }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const double v = (double)tot2[s] + (double)acc[s];
-        const double r = l3_started ? l3[s] + v : v;
-        l3[s] = act ? r : 0.0;
-    }
+    l3_accumulate<NS, 8>(l3, l3_started, act, [&](int s) { return (double)tot2[s] + (double)acc[s]; });
     qtot += (double)qacc;
     const double ls = wave_sum_f64(qtot);
     if (lane == 0) llw[w] = ls;
