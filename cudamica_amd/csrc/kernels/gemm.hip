@@ -1142,10 +1142,11 @@ static bool gx_f32_split(int NP, int ldb) {
 }
 
 int gx_f32_chunks(long T, int NP, int ldb, int cus) {
-    /* chunks of >= 512 samples per wave (S of them where the waves split a
-     * chunk), at most 256 MB of partials, and at least one per CU of the
-     * budget where T allows */
-    const long per = gx_f32_split(NP, ldb) ? 512L * (8 / (ldb / 64)) : 512L;
+    /* chunks of >= 512 samples (>= 256 per wave where S waves split a
+     * chunk: twice the waves in flight for the same partial count), at
+     * most 256 MB of partials, and at least one per CU of the budget where
+     * T allows */
+    const long per = gx_f32_split(NP, ldb) ? 256L * (8 / (ldb / 64)) : 512L;
     const long by_len = (T + per - 1) / per;
     const long by_mem = std::max(1L, (256L << 20) / ((long)ldb * NP * 8));
     const int slabs = NP / (16 * gx_f32_ct(NP));
