@@ -589,42 +589,75 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-template <int CTRL>
-__device__ __forceinline__ void argmax_step(double& bv, int& bi) {
-    const double ov = dpp_f64<CTRL>(bv);
-    const int oi = __builtin_amdgcn_update_dpp(0, bi, CTRL, 0xf, 0xf, false);
-    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+/* 32-bit maximum over the wave, in every lane: row maxima by DPP (quad
+ * permutes, then the half-row and row mirrors), then the four rows' */
+template <typename I>
+__device__ __forceinline__ I wave_max_bcast(I v) {
+    auto dmax = [](I x, auto ctrl) {
+        const I o = (I)__builtin_amdgcn_mov_dpp((int)x, decltype(ctrl)::value, 0xf, 0xf, false);
+        return o > x ? o : x;
+    };
+    v = dmax(v, std::integral_constant<int, 0xB1>());
+    v = dmax(v, std::integral_constant<int, 0x4E>());
+    v = dmax(v, std::integral_constant<int, 0x141>());
+    v = dmax(v, std::integral_constant<int, 0x140>());
+    const I r0 = (I)__builtin_amdgcn_readlane((int)v, 0), r1 = (I)__builtin_amdgcn_readlane((int)v, 16);
+    const I r2 = (I)__builtin_amdgcn_readlane((int)v, 32), r3 = (I)__builtin_amdgcn_readlane((int)v, 48);
+    const I m01 = r0 > r1 ? r0 : r1, m23 = r2 > r3 ? r2 : r3;
+    return m01 > m23 ? m01 : m23;
 }
 
-/* (largest value, smallest index among equals) over the wave, uniform */
-__device__ __forceinline__ int wave_argmax(double bv, int bi) {
-    argmax_step<0xB1>(bv, bi);      /* quad_perm [1,0,3,2] */
-    argmax_step<0x4E>(bv, bi);      /* quad_perm [2,3,0,1] */
-    argmax_step<0x141>(bv, bi);     /* row_half_mirror: the other quad of 8 */
-    argmax_step<0x140>(bv, bi);     /* row_mirror: the other 8 of the row */
-    double best = readlane_f64(bv, 0);
-    int besti = __builtin_amdgcn_readlane(bi, 0);
+/* Pivot row of the column held as col[r] (rows lane + 64 r): the largest
+ * |value| among the unused rows (st < 0; NaN counts as infinite, so that a
+ * NaN column still names an unused row), ties to the smallest row index;
+ * wave-uniform.  The bit patterns of non-negative doubles order like their
+ * values, so the maximum is found on the high words (used rows -1) and,
+ * only where several rows share the largest high word, on their low words:
+ * 32-bit DPP maxima instead of 64-bit value-and-index compares. */
+template <int R>
+__device__ __forceinline__ int gj_pivot(const double (&col)[R], const int (&st)[R]) {
+    int hi[R];
+    unsigned lo[R];
+    int mh = -1;
 #pragma unroll
-    for (int row = 1; row < 4; ++row) {
-        const double ov = readlane_f64(bv, 16 * row);
-        const int oi = __builtin_amdgcn_readlane(bi, 16 * row);
-        if (ov > best || (ov == best && oi < besti)) { best = ov; besti = oi; }
+    for (int r = 0; r < R; ++r) {
+        double x = fabs(col[r]);
+        if (x != x) x = __builtin_inf();
+        const uint64_t u = __builtin_bit_cast(uint64_t, x);
+        hi[r] = st[r] < 0 ? (int)(u >> 32) : -1;
+        lo[r] = (unsigned)u;
+        mh = max(mh, hi[r]);
     }
-    return besti;
+    mh = wave_max_bcast<int>(mh);
+    unsigned long long cand[R];
+    int nc = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        cand[r] = __ballot(hi[r] == mh);
+        nc += __builtin_popcountll(cand[r]);
+    }
+    if (nc > 1) {                           /* equal high words: compare the low words */
+        unsigned ml = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (hi[r] == mh) ml = max(ml, lo[r]);
+        ml = wave_max_bcast<unsigned>(ml);
+#pragma unroll
+        for (int r = 0; r < R; ++r) cand[r] = __ballot(hi[r] == mh && lo[r] == ml);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (cand[r]) return 64 * r + __builtin_ctzll(cand[r]);
+    return 0;       /* not reached: an unused row holds the maximum */
 }
 
 /* argmax of |v[.][c]| over this wave's unused rows (the pivot of column c) */
 template <int R, int NB>
 __device__ __forceinline__ int gj_col_argmax(const double (&v)[R][NB], const int (&st)[R], int c) {
-    double bv = -1.0;
-    int bi = 0x7fffffff;
+    double col[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        double av = fabs(v[r][c]);
-        if (av != av) av = __builtin_inf();         /* a NaN column still names an unused row */
-        if (st[r] < 0 && av > bv) { bv = av; bi = (int)threadIdx.x + 64 * r; }
-    }
-    return wave_argmax(bv, bi);
+    for (int r = 0; r < R; ++r) col[r] = v[r][c];
+    return gj_pivot<R>(col, st);
 }
 
 /* NB Gauss-Jordan steps on the panel v (columns k0 .., rows lane + 64 r),
@@ -871,67 +904,6 @@ __global__ __launch_bounds__(1024) void gj_gather_kernel(MstepArgs a) {
         for (int w = 0; w < nw; ++w) lad += lsum[w];
         set_ldet(a, h, lad);
     }
-}
-
-/* 32-bit maximum over the wave, in every lane: row maxima by DPP (quad
- * permutes, then the half-row and row mirrors), then the four rows' */
-template <typename I>
-__device__ __forceinline__ I wave_max_bcast(I v) {
-    auto dmax = [](I x, auto ctrl) {
-        const I o = (I)__builtin_amdgcn_mov_dpp((int)x, decltype(ctrl)::value, 0xf, 0xf, false);
-        return o > x ? o : x;
-    };
-    v = dmax(v, std::integral_constant<int, 0xB1>());
-    v = dmax(v, std::integral_constant<int, 0x4E>());
-    v = dmax(v, std::integral_constant<int, 0x141>());
-    v = dmax(v, std::integral_constant<int, 0x140>());
-    const I r0 = (I)__builtin_amdgcn_readlane((int)v, 0), r1 = (I)__builtin_amdgcn_readlane((int)v, 16);
-    const I r2 = (I)__builtin_amdgcn_readlane((int)v, 32), r3 = (I)__builtin_amdgcn_readlane((int)v, 48);
-    const I m01 = r0 > r1 ? r0 : r1, m23 = r2 > r3 ? r2 : r3;
-    return m01 > m23 ? m01 : m23;
-}
-
-/* Pivot row of the column held as col[r] (rows lane + 64 r): the largest
- * |value| among the unused rows (st < 0; NaN counts as infinite), ties to
- * the smallest row index -- the rule of gj_col_argmax; wave-uniform.  The
- * bit patterns of non-negative doubles order like their values, so the
- * maximum is found on the high words (used rows -1) and, only where several
- * rows share the largest high word, on their low words. */
-template <int R>
-__device__ __forceinline__ int gj_pivot(const double (&col)[R], const int (&st)[R]) {
-    int hi[R];
-    unsigned lo[R];
-    int mh = -1;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        double x = fabs(col[r]);
-        if (x != x) x = __builtin_inf();
-        const uint64_t u = __builtin_bit_cast(uint64_t, x);
-        hi[r] = st[r] < 0 ? (int)(u >> 32) : -1;
-        lo[r] = (unsigned)u;
-        mh = max(mh, hi[r]);
-    }
-    mh = wave_max_bcast<int>(mh);
-    unsigned long long cand[R];
-    int nc = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        cand[r] = __ballot(hi[r] == mh);
-        nc += __builtin_popcountll(cand[r]);
-    }
-    if (nc > 1) {                           /* equal high words: compare the low words */
-        unsigned ml = 0;
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (hi[r] == mh) ml = max(ml, lo[r]);
-        ml = wave_max_bcast<unsigned>(ml);
-#pragma unroll
-        for (int r = 0; r < R; ++r) cand[r] = __ballot(hi[r] == mh && lo[r] == ml);
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-        if (cand[r]) return 64 * r + __builtin_ctzll(cand[r]);
-    return 0;       /* not reached: an unused row holds the maximum */
 }
 
 /* n <= 192: the whole inverse of one model by one workgroup, the matrix in
