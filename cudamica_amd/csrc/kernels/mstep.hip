@@ -94,7 +94,8 @@ __global__ __launch_bounds__(64 * kSumWaves) void sum_parts_kernel(const double*
 }
 
 /* Canonical statistics of model blockIdx.x (layout of the oracle, see the
- * engine header); blockIdx.y splits the phi rows. */
+ * engine header), by the workgroups of blockIdx.y = 0; blockIdx.y is the
+ * 16 x 16 tile of phi. */
 __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict__ raw,
                                                         const double* __restrict__ gx,
                                                         MstepArgs a) {
@@ -141,45 +142,38 @@ __global__ __launch_bounds__(kBlk) void finalize_kernel(const double* __restrict
         }
     }
     /* phi_h[i][k] = sum_l G[h*NP+i][l] W~_h[k][l] - gsum_h[i] wc_h[k]: one
-     * wave per (i, k), lanes over l (coalesced rows of W~), two k at a time */
+     * 16 x 16 (i, k) tile per workgroup (blockIdx.y), the l sum through
+     * 16-wide LDS tiles of G and W~ (W~ joined to FP64 as it is staged: the
+     * unmixing GEMM's W_hi + W_lo, fp32(W) or W), one FMA per l in
+     * increasing l */
     const double* gsum_base = rh + kStatsPerJ * m + kRawGsum;
-    const int rows_per = (n + gridDim.y - 1) / gridDim.y;
-    const int ib = blockIdx.y * rows_per, ie = min(n, ib + rows_per);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int i = ib; i < ie; ++i) {
-        const double* grow = gx + ((size_t)h * NP + i) * NP;
-        const double gs = gsum_base[(size_t)i * NS];
-        for (int k0 = 2 * wv; k0 < n; k0 += 2 * nw) {
-            const int k1 = min(k0 + 1, n - 1);
-            const size_t w0 = ((size_t)h * NP + k0) * NP, w1 = ((size_t)h * NP + k1) * NP;
-            double s0 = 0.0, s1 = 0.0;
-            for (int l = lane; l < n; l += 64) {
-                const double g = grow[l];
-                if (a.prec_bf16) {
-                    /* the unmixing GEMM used W~ = W_hi + W_lo */
-                    s0 += g * ((double)bf16_bits_to_f32(a.Whi[w0 + l]) + (double)bf16_bits_to_f32(a.Wlo[w0 + l]));
-                    s1 += g * ((double)bf16_bits_to_f32(a.Whi[w1 + l]) + (double)bf16_bits_to_f32(a.Wlo[w1 + l]));
-                } else if (a.Wg32) {
-                    /* the plain FP32 GEMM used W~ = fp32(W) */
-                    s0 += g * (double)a.Wg32[w0 + l];
-                    s1 += g * (double)a.Wg32[w1 + l];
-                } else {
-                    s0 += g * a.Wg64[w0 + l];
-                    s1 += g * a.Wg64[w1 + l];
-                }
-            }
-            s0 = wave_sum_f64(s0);
-            s1 = wave_sum_f64(s1);
-            if (lane == 0) {
-                const bool wcf = a.prec_bf16 || a.Wg32;   /* W~ c held in FP32 */
-                const double wc0 = wcf ? (double)a.wc[(size_t)h * NP + k0] : a.wc64[(size_t)h * NP + k0];
-                st[o.phi + i + (long)k0 * n] = s0 - gs * wc0;
-                if (k1 != k0) {
-                    const double wc1 = wcf ? (double)a.wc[(size_t)h * NP + k1] : a.wc64[(size_t)h * NP + k1];
-                    st[o.phi + i + (long)k1 * n] = s1 - gs * wc1;
-                }
-            }
+    const int ntk = (n + 15) / 16;
+    const int tib = (int)blockIdx.y / ntk, tkb = (int)blockIdx.y - tib * ntk;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    __shared__ double Gs[16][17];                  /* [i][l] */
+    __shared__ double Ws[16][17];                  /* [k][l] */
+    double acc = 0.0;
+    for (int l0 = 0; l0 < n; l0 += 16) {
+        const int l = l0 + tx, gi = tib * 16 + ty, wk = tkb * 16 + ty;
+        Gs[ty][tx] = (gi < n && l < n) ? gx[((size_t)h * NP + gi) * NP + l] : 0.0;
+        double wv = 0.0;
+        if (wk < n && l < n) {
+            const size_t wi = ((size_t)h * NP + wk) * NP + l;
+            if (a.prec_bf16) wv = (double)bf16_bits_to_f32(a.Whi[wi]) + (double)bf16_bits_to_f32(a.Wlo[wi]);
+            else if (a.Wg32) wv = (double)a.Wg32[wi];
+            else wv = a.Wg64[wi];
         }
+        Ws[ty][tx] = wv;
+        __syncthreads();
+        const int lend = min(16, n - l0);
+        for (int q = 0; q < lend; ++q) acc += Gs[ty][q] * Ws[tx][q];
+        __syncthreads();
+    }
+    const int i = tib * 16 + ty, k = tkb * 16 + tx;
+    if (i < n && k < n) {
+        const bool wcf = a.prec_bf16 || a.Wg32;       /* W~ c held in FP32 */
+        const double wck = wcf ? (double)a.wc[(size_t)h * NP + k] : a.wc64[(size_t)h * NP + k];
+        st[o.phi + i + (long)k * n] = acc - gsum_base[(size_t)i * NS] * wck;
     }
 }
 
@@ -1140,8 +1134,9 @@ int launch_reduce(const double* part, int nparts, size_t pstride, const double* 
     sum_parts(part, nparts, pstride, plen, tmp, raw, st);
     const size_t glen = (size_t)ma.ldb * ma.NP;
     sum_parts(gxpart, nchunks, glen, glen, tmp, gxtot, st);
-    /* one phi row per workgroup (threads over its columns) */
-    hipLaunchKernelGGL(finalize_kernel, dim3(ma.M, ma.n), dim3(kBlk), 0, st, raw, gxtot, ma);
+    /* one 16 x 16 tile of phi per workgroup (the statistics by the first) */
+    const unsigned ntk = (unsigned)((ma.n + 15) / 16);
+    hipLaunchKernelGGL(finalize_kernel, dim3(ma.M, ntk * ntk), dim3(kBlk), 0, st, raw, gxtot, ma);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
